@@ -1,0 +1,503 @@
+// MFT Poisson solver on gfx950: rho -> phi -> E.
+//
+// Restates MFT_solve (src/solver.c:466-509): a 2D real-to-complex FFT of rho
+// (ny x nx, X fastest), multiplication by the Green's function
+// G(kx,ky) = 1/(2(cos 2pi kx/nx + cos 2pi ky/ny) - 4), G(0,0) = 0
+// (src/solver.c:269-280), the complex-to-real inverse and the division by
+// nx*ny (src/solver.c:366-379), then E = -grad(phi) by periodic central
+// differences (field_E_compute, src/field.c:359-416).
+//
+// Three LDS passes, all HBM-bound:
+//   1. k_rows_r2c   R rows per workgroup: half-length complex FFT of each
+//                   row + real post-processing; writes ghat kx-major
+//                   (transposed) so that R consecutive ky are contiguous.
+//   2. k_cols       one or more kx columns per workgroup: forward FFT along
+//                   ky, multiply by G/(nx*ny) (the normalisation is folded in;
+//                   nx*ny is a power of two so the scaling is exact), the field
+//                   energy partial sum, inverse FFT along ky.
+//   3. k_rows_c2r   R rows per workgroup: Hermitian pre-processing + inverse
+//                   half-length FFT; writes phi rows.
+// then k_efield writes (Ex,Ey) pairs in the particle kernels' precision and
+// zeroes the rho buffer the next push deposits into.
+//
+// FFT stages are radix-2 in LDS with twiddles from one table of
+// exp(-2 pi i k/N), N = max(nx, ny), built on the host with an octant
+// reduction so the quarter points are exact.
+#include "dev.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FFT_LDS_BYTES (64 * 1024)
+
+static __device__ __forceinline__ int brev(int j, int lg)
+{
+	return lg ? (int) (__brev((unsigned) j) >> (32 - lg)) : 0;
+}
+
+static __device__ __forceinline__ double2 cmul(double2 a, double2 w)
+{
+	return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
+}
+
+// Decimation-in-time FFT of nbat transforms of length n = 2^lgn stored at
+// buf + b*stride; input in bit-reversed order, output natural.  w = exp(sign
+// 2 pi i k/len) taken from tw (size 2^lgN table).  Ends with a barrier.
+static __device__ void fft_dit(double2 *buf, int stride, int lgn, int nbat,
+		const double2 *__restrict__ tw, int lgN, double sign)
+{
+	int n = 1 << lgn;
+	int nbf = (n >> 1) * nbat;
+	for(int s = 1; s <= lgn; s++)
+	{
+		int half = 1 << (s - 1);
+		int tsh = lgN - s;
+		for(int t = threadIdx.x; t < nbf; t += blockDim.x)
+		{
+			int bat = t >> (lgn - 1);
+			int bf = t & ((n >> 1) - 1);
+			int k = bf & (half - 1);
+			int i0 = bat * stride + ((bf >> (s - 1)) << s) + k;
+			int i1 = i0 + half;
+			double2 w = tw[k << tsh];
+			w.y *= sign;
+			double2 a = buf[i0];
+			double2 v = cmul(buf[i1], w);
+			buf[i0] = make_double2(a.x + v.x, a.y + v.y);
+			buf[i1] = make_double2(a.x - v.x, a.y - v.y);
+		}
+		__syncthreads();
+	}
+}
+
+// Decimation-in-frequency FFT: natural input, bit-reversed output.
+static __device__ void fft_dif(double2 *buf, int stride, int lgn, int nbat,
+		const double2 *__restrict__ tw, int lgN, double sign)
+{
+	int n = 1 << lgn;
+	int nbf = (n >> 1) * nbat;
+	for(int s = lgn; s >= 1; s--)
+	{
+		int half = 1 << (s - 1);
+		int tsh = lgN - s;
+		for(int t = threadIdx.x; t < nbf; t += blockDim.x)
+		{
+			int bat = t >> (lgn - 1);
+			int bf = t & ((n >> 1) - 1);
+			int k = bf & (half - 1);
+			int i0 = bat * stride + ((bf >> (s - 1)) << s) + k;
+			int i1 = i0 + half;
+			double2 w = tw[k << tsh];
+			w.y *= sign;
+			double2 a = buf[i0], b = buf[i1];
+			buf[i0] = make_double2(a.x + b.x, a.y + b.y);
+			buf[i1] = cmul(make_double2(a.x - b.x, a.y - b.y), w);
+		}
+		__syncthreads();
+	}
+}
+
+// Pass 1: R rows per block; row y real of length nx = 2m.
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_rows_r2c(const double *__restrict__ rho, double2 *__restrict__ ghat,
+		int nx, int ny, int R, const double2 *__restrict__ tw, int lgN)
+{
+	extern __shared__ double2 sh[];
+	int m = nx >> 1, lgm = 0;
+	while((1 << lgm) < m)
+		lgm++;
+	int stride = m + 1;
+	int y0 = blockIdx.x * R;
+
+	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
+	{
+		int r = i / m, j = i - r * m;
+		const double *row = rho + (int64_t) (y0 + r) * nx;
+		double2 z = make_double2(row[2 * j], row[2 * j + 1]);
+		sh[r * stride + brev(j, lgm)] = z;
+	}
+	__syncthreads();
+	if(lgm > 0)
+		fft_dit(sh, stride, lgm, R, tw, lgN, -1.0);
+
+	// X[k] = E + W^k O, X[m-k] = conj(E - W^k O), W = exp(-2 pi i/nx)
+	int npair = (m >> 1) + 1;
+	int tsh = lgN - (lgm + 1);
+	for(int i = threadIdx.x; i < R * npair; i += blockDim.x)
+	{
+		int r = i / npair, k = i - r * npair;
+		double2 *b = sh + r * stride;
+		if(k == 0)
+		{
+			double2 z = b[0];
+			b[0] = make_double2(z.x + z.y, 0.0);
+			b[m] = make_double2(z.x - z.y, 0.0);
+		}
+		else if(2 * k == m)
+		{
+			double2 z = b[k];
+			b[k] = make_double2(z.x, -z.y);
+		}
+		else
+		{
+			double2 zk = b[k], zm = b[m - k];
+			// E = (zk + conj zm)/2, O = (zk - conj zm)/(2i)
+			double er = 0.5 * (zk.x + zm.x), ei = 0.5 * (zk.y - zm.y);
+			double dr = 0.5 * (zk.x - zm.x), di = 0.5 * (zk.y + zm.y);
+			double2 O = make_double2(di, -dr);
+			double2 w = tw[k << tsh];
+			w.y = -w.y;
+			double2 wo = cmul(O, w);
+			b[k] = make_double2(er + wo.x, ei + wo.y);
+			b[m - k] = make_double2(er - wo.x, -(ei - wo.y));
+		}
+	}
+	__syncthreads();
+
+	// transposed store: ghat[k*ny + y]
+	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
+	{
+		int k = i / R, r = i - k * R;
+		ghat[(int64_t) k * ny + y0 + r] = sh[r * stride + k];
+	}
+}
+
+// Pass 2: C columns (kx) per block, each ny long and contiguous.
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_cols(double2 *__restrict__ ghat, int nx, int ny, int C,
+		const double *__restrict__ costab, const double2 *__restrict__ tw,
+		int lgN, double invN, double *__restrict__ energy, double escale)
+{
+	extern __shared__ double2 sh[];
+	__shared__ double red[CPIC_BLOCK];
+	int m = nx >> 1;
+	int lgny = 0;
+	while((1 << lgny) < ny)
+		lgny++;
+	int k0 = blockIdx.x * C;
+	int ncol = min(C, m + 1 - k0);
+	const double *cosx = costab, *cosy = costab + (m + 1);
+
+	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
+	{
+		int c = i >> lgny, j = i & (ny - 1);
+		sh[c * ny + brev(j, lgny)] = ghat[(int64_t) (k0 + c) * ny + j];
+	}
+	__syncthreads();
+	if(lgny > 0)
+		fft_dit(sh, ny, lgny, ncol, tw, lgN, -1.0);
+
+	double e = 0.0;
+	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
+	{
+		int c = i >> lgny, ky = i & (ny - 1);
+		int kx = k0 + c;
+		// G exactly as MFT_init computes it
+		double G = (kx == 0 && ky == 0) ? 0.0 :
+			1.0 / (2.0 * (cosx[kx] + cosy[ky]) - 4.0);
+		double2 v = sh[i];
+		double wk = (kx == 0 || kx == m) ? 1.0 : 2.0;
+		e += wk * G * (v.x * v.x + v.y * v.y);
+		double g = G * invN;
+		sh[i] = make_double2(v.x * g, v.y * g);
+	}
+	if(energy)
+	{
+		red[threadIdx.x] = e;
+		__syncthreads();
+		for(int s = CPIC_BLOCK / 2; s > 0; s >>= 1)
+		{
+			if((int) threadIdx.x < s)
+				red[threadIdx.x] += red[threadIdx.x + s];
+			__syncthreads();
+		}
+		if(threadIdx.x == 0)
+			unsafeAtomicAdd(energy, red[0] * escale);
+	}
+	__syncthreads();
+	if(lgny > 0)
+		fft_dif(sh, ny, lgny, ncol, tw, lgN, +1.0);
+
+	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
+	{
+		int c = i >> lgny, j = i & (ny - 1);
+		ghat[(int64_t) (k0 + c) * ny + j] = sh[c * ny + brev(j, lgny)];
+	}
+}
+
+// Pass 3: R rows per block, c2r along X (imaginary parts of X[0] and
+// X[nx/2] ignored, as for a Hermitian input).
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_rows_c2r(const double2 *__restrict__ ghat, double *__restrict__ phi,
+		int nx, int ny, int R, const double2 *__restrict__ tw, int lgN)
+{
+	extern __shared__ double2 sh[];
+	int m = nx >> 1, lgm = 0;
+	while((1 << lgm) < m)
+		lgm++;
+	int stride = m + 1;
+	int y0 = blockIdx.x * R;
+
+	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
+	{
+		int k = i / R, r = i - k * R;
+		sh[r * stride + k] = ghat[(int64_t) k * ny + y0 + r];
+	}
+	__syncthreads();
+
+	// Z[k] = A + iB, A = X[k] + conj X[m-k], B = (X[k] - conj X[m-k]) e^{+2 pi i k/nx}
+	int npair = (m >> 1) + 1;
+	int tsh = lgN - (lgm + 1);
+	for(int i = threadIdx.x; i < R * npair; i += blockDim.x)
+	{
+		int r = i / npair, k = i - r * npair;
+		double2 *b = sh + r * stride;
+		if(k == 0)
+		{
+			double a0 = b[0].x, am = b[m].x;
+			b[0] = make_double2(a0 + am, a0 - am);
+		}
+		else if(2 * k == m)
+		{
+			// A = (2 Re x, 0), D = (0, 2 Im x), B = D e^{i pi/2} = (-2 Im x, 0),
+			// Z = A + iB
+			double2 x = b[k];
+			b[k] = make_double2(2.0 * x.x, -2.0 * x.y);
+		}
+		else
+		{
+			double2 xk = b[k], xm = b[m - k];
+			double2 wk = tw[k << tsh];          // e^{+2 pi i k/nx} = (c, s)
+			double2 wm = tw[(m - k) << tsh];    // e^{+2 pi i (m-k)/nx}
+			// for k
+			double2 A = make_double2(xk.x + xm.x, xk.y - xm.y);
+			double2 D = make_double2(xk.x - xm.x, xk.y + xm.y);
+			double2 B = cmul(D, wk);
+			// for m-k
+			double2 A2 = make_double2(xm.x + xk.x, xm.y - xk.y);
+			double2 D2 = make_double2(xm.x - xk.x, xm.y + xk.y);
+			double2 B2 = cmul(D2, wm);
+			b[k] = make_double2(A.x - B.y, A.y + B.x);
+			b[m - k] = make_double2(A2.x - B2.y, A2.y + B2.x);
+		}
+	}
+	__syncthreads();
+	if(lgm > 0)
+		fft_dif(sh, stride, lgm, R, tw, lgN, +1.0);
+
+	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
+	{
+		int r = i / m, j = i - r * m;
+		double2 z = sh[r * stride + brev(j, lgm)];
+		double *row = phi + (int64_t) (y0 + r) * nx;
+		row[2 * j] = z.x;
+		row[2 * j + 1] = z.y;
+	}
+}
+
+// E = -grad(phi) (src/field.c:381-406) and zeroing of the next rho buffer.
+template <typename T2>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_efield(const double *__restrict__ phi, T2 *__restrict__ E, int nx, int ny,
+		double dx2, double dy2, double *__restrict__ rho_zero,
+		DiagSlot *__restrict__ diag_zero)
+{
+	int64_t n = (int64_t) nx * ny;
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+			i += (int64_t) gridDim.x * blockDim.x)
+	{
+		int iy = (int) (i / nx), ix = (int) (i - (int64_t) iy * nx);
+		int x0 = ix == 0 ? nx - 1 : ix - 1, x1 = ix == nx - 1 ? 0 : ix + 1;
+		int y0 = iy == 0 ? ny - 1 : iy - 1, y1 = iy == ny - 1 ? 0 : iy + 1;
+		const double *row = phi + (int64_t) iy * nx;
+		double ey = (phi[(int64_t) y0 * nx + ix] - phi[(int64_t) y1 * nx + ix]) / dy2;
+		double ex = (row[x0] - row[x1]) / dx2;
+		T2 v;
+		v.x = ex;
+		v.y = ey;
+		E[i] = v;
+		if(rho_zero)
+			rho_zero[i] = 0.0;
+	}
+	if(diag_zero && blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		diag_zero->field = 0.0;
+		diag_zero->kinetic = 0.0;
+		diag_zero->px = 0.0;
+		diag_zero->py = 0.0;
+	}
+}
+
+// ---------------------------------------------------------------- host ---
+
+static void
+twiddle(int64_t k, int64_t n, double *c, double *s)
+{
+	int64_t k8 = 8 * k;
+	double a;
+	if(k8 <= n)
+	{
+		a = 2.0 * M_PI * (double) k / (double) n;
+		*c = cos(a); *s = sin(a);
+	}
+	else if(k8 <= 2 * n)
+	{
+		a = 2.0 * M_PI * (double) (n / 4 - k) / (double) n;
+		*c = sin(a); *s = cos(a);
+	}
+	else if(k8 <= 3 * n)
+	{
+		a = 2.0 * M_PI * (double) (k - n / 4) / (double) n;
+		*c = -sin(a); *s = cos(a);
+	}
+	else
+	{
+		a = 2.0 * M_PI * (double) (n / 2 - k) / (double) n;
+		*c = -cos(a); *s = sin(a);
+	}
+}
+
+int
+cpic_fft_init(cpic_dev *d)
+{
+	int64_t nx = d->g.nx, ny = d->g.ny;
+	int64_t N = nx > ny ? nx : ny;
+	int64_t hx = nx / 2 + 1, k;
+	double cx = 2.0 * M_PI / nx, cy = 2.0 * M_PI / ny;
+	double *ct;
+	double2 *tw;
+
+	d->lgN = ilog2(N);
+	ct = (double *) malloc(sizeof(double) * (size_t) (hx + ny));
+	tw = (double2 *) malloc(sizeof(double2) * (size_t) (N / 2 + 1));
+	if(!ct || !tw)
+	{
+		free(ct); free(tw);
+		return CPIC_ENOMEM;
+	}
+	// MFT_init's cosines (src/solver.c:258-280), bit-identical
+	for(k = 0; k < hx; k++)
+		ct[k] = cos(cx * k);
+	for(k = 0; k < ny; k++)
+		ct[hx + k] = cos(cy * k);
+	for(k = 0; k <= N / 2; k++)
+		twiddle(k, N, &tw[k].x, &tw[k].y);
+
+	if(hipMalloc(&d->costab, sizeof(double) * (size_t) (hx + ny)) != hipSuccess ||
+			hipMalloc(&d->tw, sizeof(double2) * (size_t) (N / 2 + 1)) != hipSuccess)
+	{
+		free(ct); free(tw);
+		return CPIC_ENOMEM;
+	}
+	hipError_t e1 = hipMemcpy(d->costab, ct, sizeof(double) * (size_t) (hx + ny),
+			hipMemcpyHostToDevice);
+	hipError_t e2 = hipMemcpy(d->tw, tw, sizeof(double2) * (size_t) (N / 2 + 1),
+			hipMemcpyHostToDevice);
+	free(ct);
+	free(tw);
+	if(e1 != hipSuccess || e2 != hipSuccess)
+		return CPIC_EHIP;
+	// one row of 8192 points needs 64 KB + 16 B of LDS, a column 128 KB
+	HIPCHK(hipFuncSetAttribute((const void *) k_rows_r2c,
+			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
+	HIPCHK(hipFuncSetAttribute((const void *) k_rows_c2r,
+			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
+	HIPCHK(hipFuncSetAttribute((const void *) k_cols,
+			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
+	return 0;
+}
+
+static int
+rows_per_block(int64_t nx, int64_t ny)
+{
+	int64_t per = (nx / 2 + 1) * (int64_t) sizeof(double2);
+	int64_t R = FFT_LDS_BYTES / per;
+	int64_t want = 8;
+	if(R > want)
+		R = want;
+	if(R > ny)
+		R = ny;
+	// power of two dividing ny
+	int64_t r = 1;
+	while(r * 2 <= R)
+		r *= 2;
+	return (int) r;
+}
+
+int
+cpic_solve(cpic_dev *d, int64_t iter, int want_energy)
+{
+	int nx = d->g.nx, ny = d->g.ny, m = nx / 2;
+	int R = rows_per_block(nx, ny);
+	int C;
+	int slot;
+	size_t lds;
+	double *energy = NULL;
+	double N = (double) nx * (double) ny;
+
+	// pass 1
+	lds = sizeof(double2) * (size_t) R * (size_t) (m + 1);
+	slot = cpic_kt_begin(d, "fft_rows_r2c");
+	hipLaunchKernelGGL(k_rows_r2c, dim3(ny / R), dim3(CPIC_BLOCK), lds, d->stream,
+			d->rho[d->rho_cur], d->ghat, nx, ny, R, d->tw, d->lgN);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+
+	// pass 2
+	C = (int) (FFT_LDS_BYTES / ((int64_t) ny * (int64_t) sizeof(double2)));
+	if(C < 1)
+		C = 1;
+	if(C > m + 1)
+		C = m + 1;
+	if(C > 16)
+		C = 16;
+	lds = sizeof(double2) * (size_t) C * (size_t) ny;
+	if(want_energy && iter >= 0 && !(d->desc.flags & CPIC_F_NO_DIAG))
+		energy = &d->diag[iter % CPIC_DIAG_RING].field;
+	slot = cpic_kt_begin(d, "fft_cols_green");
+	hipLaunchKernelGGL(k_cols, dim3((m + 1 + C - 1) / C), dim3(CPIC_BLOCK), lds, d->stream,
+			d->ghat, nx, ny, C, d->costab, d->tw, d->lgN, 1.0 / N, energy,
+			-0.5 * d->desc.e0 / N);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+
+	// pass 3
+	lds = sizeof(double2) * (size_t) R * (size_t) (m + 1);
+	slot = cpic_kt_begin(d, "fft_rows_c2r");
+	hipLaunchKernelGGL(k_rows_c2r, dim3(ny / R), dim3(CPIC_BLOCK), lds, d->stream,
+			d->ghat, d->phi, nx, ny, R, d->tw, d->lgN);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int
+cpic_efield(cpic_dev *d, int64_t iter)
+{
+	int nx = d->g.nx, ny = d->g.ny;
+	int64_t n = (int64_t) nx * ny;
+	int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+	double *rz = NULL;
+	DiagSlot *dz = NULL;
+	int slot;
+
+	if(blocks > 8192)
+		blocks = 8192;
+	if(!d->rho_next_ready && !d->rho_pending)
+		rz = d->rho[1 - d->rho_cur];
+	if(!(d->desc.flags & CPIC_F_NO_DIAG))
+		dz = &d->diag[(iter + 1) % CPIC_DIAG_RING];
+	slot = cpic_kt_begin(d, "efield");
+	if(d->prec == 64)
+		hipLaunchKernelGGL(k_efield<double2>, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->phi, (double2 *) d->E, nx, ny, 2 * d->g.dx, 2 * d->g.dy, rz, dz);
+	else
+		hipLaunchKernelGGL(k_efield<float2>, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->phi, (float2 *) d->E, nx, ny, 2 * d->g.dx, 2 * d->g.dy, rz, dz);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	if(rz)
+		d->rho_next_ready = 1;
+	return 0;
+}

@@ -1,0 +1,1244 @@
+// Particle kernels of the gfx950 PIC step.
+//
+// k_push: one workgroup per tile.  Stages the tile's E region (tile + 1 cell
+// margin + the CIC +1 node) in LDS, streams every particle of the tile's
+// buckets once (coalesced SoA loads), and for each particle
+//   gathers E with CIC weights    (interpolate_f2p_E, src/interpolate.c:350-404)
+//   applies the Boris push        (boris_rotation/move, src/mover.c:23-83, 141-227)
+//   wraps periodically            (periodic_boundary_ppack, src/comm_plasma.c:726-747)
+//   deposits -q/e0 * w into an LDS rho region (interpolate_p2f_rho,
+//                                  src/interpolate.c:283-346; FP64 LDS atomics in
+//                                  FP64 mode, FP32 LDS atomics in FP32 mode)
+//   and compacts the tile's stayers in place while leavers (particles whose
+//   cell moved to another tile: comm_plasma's collect, src/comm_plasma.c:
+//   101-165) are appended to a leaver buffer through one wave-aggregated
+//   atomic per wavefront (ballot + popcount).
+// The LDS rho region is flushed once per tile with FP64 global atomics.
+// k_append then places each leaver at the end of its destination bucket
+// (inject_particles, src/comm_plasma.c:358-400) or, when that bucket is full,
+// back into a reserved slot of its source bucket, so no particle is ever lost
+// whatever the capacities; such particles are handled by the slow global
+// path until the next re-sort.
+//
+// FP64 mode uses the reference's arithmetic bit for bit: positions are
+// absolute, cell indices and weights follow relative_position_grid
+// (src/interpolate.c:39-66, including dx[X] in the Y term) and the Boris
+// push is the reference's 3D form reduced exactly to B = (0,0,Bz)
+// (u_z stays 0).  The translation unit is compiled with -ffp-contract=off;
+// the multiply-adds the reference's clang build contracts are explicit fma().
+#include "dev.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+template <typename T> struct V2;
+template <> struct V2<double> { typedef double2 t; };
+template <> struct V2<float> { typedef float2 t; };
+
+struct PushArgs {
+	Geo g;
+	Species sp;
+	Buckets bk;
+	Parts p;
+	Leavers lv;
+	int par;              // leaver counter parity
+	const void *E;
+	double *rho;          // deposit target
+	DiagSlot *diag;       // nullable
+	int *derr;
+	Counters *ctr;
+	int set_r;            // 0 on iteration 0 (no position update)
+	int mode;             // PUSH / DEPOSIT / GATHER
+	double dtp;           // dt' (dt, or -dt/2 on iteration 0)
+	double umax_x, umax_y;
+	float dtdx, dtdy;     // FP32 mode: dt' / dx, dt' / dy
+};
+
+enum { MODE_PUSH = 0, MODE_DEPOSIT = 1, MODE_GATHER = 2 };
+
+static __device__ __forceinline__ int wrapi(int i, int n)
+{
+	i = i < 0 ? i + n : i;
+	return i >= n ? i - n : i;
+}
+
+static __device__ __forceinline__ int tile_map(int b, int ntiles)
+{
+	// spread contiguous tile bands over the 8 XCDs (workgroups are
+	// dispatched round-robin), so neighbouring tiles share an L2
+	if((ntiles & 7) == 0)
+		return (b & 7) * (ntiles >> 3) + (b >> 3);
+	return b;
+}
+
+static __device__ __forceinline__ int bucket_n1(const Buckets &bk, int b)
+{
+	int lim = bk.cap[b] - bk.res[b];
+	int n1 = bk.n1[b];
+	return n1 < lim ? n1 : lim;
+}
+
+static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
+		int n1, int j)
+{
+	int64_t s = bk.start[b];
+	return j < n1 ? s + j : s + (bk.cap[b] - bk.res[b]) + (j - n1);
+}
+
+// FP64 cell index and relative offset (src/interpolate.c:39-66).  The Y
+// middle term uses dx (the reference passes dx[X], src/interpolate.c:88).
+static __device__ __forceinline__ void cell64(const Geo &g, double x, double y,
+		int &ix, int &iy, double &rx, double &ry)
+{
+	double bx = x * g.idx;
+	double sx = floor(bx);
+	double gx = fma(-sx, g.dx, x);
+	rx = gx * g.idx;
+	ix = (int) sx;
+	if(ix >= g.nx)
+		ix -= g.nx;
+	double by = y * g.idy;
+	double sy = floor(by);
+	double gy = fma(-sy, g.dx, y);
+	ry = gy * g.idy;
+	iy = (int) sy;
+	if(iy >= g.ny)
+		iy -= g.ny;
+}
+
+// periodic_boundary_ppack with the x == L -> 0 rule.  A displacement larger
+// than L (only possible after |u| > umax, which is flagged) is reduced with
+// floor so that every index stays inside the grid.
+static __device__ __forceinline__ double wrap64(double r, double L)
+{
+	if(r >= L)
+		r -= L;
+	else if(r < 0.0)
+	{
+		r += L;
+		if(r >= L)
+			r = 0.0;
+	}
+	if(!(r >= 0.0 && r < L))
+	{
+		r = r - floor(r / L) * L;
+		if(!(r >= 0.0 && r < L))
+			r = 0.0;
+	}
+	return r;
+}
+
+// Per-particle state in registers.
+template <typename T> struct PState {
+	T x, y, ux, uy;
+	int64_t id;
+	double Ex, Ey;
+};
+
+template <typename T>
+static __device__ __forceinline__ void load_p(const Parts &p, int64_t s, PState<T> &q,
+		bool ids)
+{
+	q.x = ((const T *) p.x)[s];
+	q.y = ((const T *) p.y)[s];
+	q.ux = ((const T *) p.ux)[s];
+	q.uy = ((const T *) p.uy)[s];
+	if(ids)
+		q.id = p.id[s];
+}
+
+template <typename T>
+static __device__ __forceinline__ void store_p(const Parts &p, int64_t s, const PState<T> &q,
+		bool ids, bool pe)
+{
+	((T *) p.x)[s] = q.x;
+	((T *) p.y)[s] = q.y;
+	((T *) p.ux)[s] = q.ux;
+	((T *) p.uy)[s] = q.uy;
+	if(ids)
+		p.id[s] = q.id;
+	if(pe)
+	{
+		p.Ex[s] = q.Ex;
+		p.Ey[s] = q.Ey;
+	}
+}
+
+template <typename T>
+static __device__ __forceinline__ void store_l(const Leavers &l, int64_t s, const PState<T> &q,
+		bool ids, bool pe, int dst, int src)
+{
+	((T *) l.x)[s] = q.x;
+	((T *) l.y)[s] = q.y;
+	((T *) l.ux)[s] = q.ux;
+	((T *) l.uy)[s] = q.uy;
+	if(ids)
+		l.id[s] = q.id;
+	if(pe)
+	{
+		l.Ex[s] = q.Ex;
+		l.Ey[s] = q.Ey;
+	}
+	l.dst[s] = dst;
+	l.src[s] = src;
+}
+
+// Position of a particle as (global cell, in-cell offset).  For FP32 the
+// stored coordinates are cell units relative to the bucket tile origin.
+struct CellPos {
+	int ix, iy;        // global cell
+	int lx, ly;        // cell relative to the LDS region origin (may be out of range)
+};
+
+// Gather E at one particle: LDS region when the 2x2 stencil is inside it,
+// else straight from HBM.  Summation order of interpolate_f2p
+// (src/interpolate.c:151-154): w00, w01, w10, w11 with fma accumulation.
+template <typename T>
+static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>::t *sE,
+		const typename V2<T>::t *__restrict__ E, const CellPos &c, T rx, T ry,
+		T &ex, T &ey)
+{
+	typedef typename V2<T>::t T2;
+	T dxw = (T) 1 - rx, dyw = (T) 1 - ry;
+	T w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
+	T2 e00, e01, e10, e11;
+	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
+	{
+		int i = c.ly * g.W + c.lx;
+		e00 = sE[i];
+		e01 = sE[i + g.W];
+		e10 = sE[i + 1];
+		e11 = sE[i + g.W + 1];
+	}
+	else
+	{
+		int i1x = c.ix + 1 == g.nx ? 0 : c.ix + 1;
+		int i1y = c.iy + 1 == g.ny ? 0 : c.iy + 1;
+		e00 = E[(int64_t) c.iy * g.nx + c.ix];
+		e01 = E[(int64_t) i1y * g.nx + c.ix];
+		e10 = E[(int64_t) c.iy * g.nx + i1x];
+		e11 = E[(int64_t) i1y * g.nx + i1x];
+	}
+	ex = w00 * e00.x;
+	ex = fma(w01, e01.x, ex);
+	ex = fma(w10, e10.x, ex);
+	ex = fma(w11, e11.x, ex);
+	ey = w00 * e00.y;
+	ey = fma(w01, e01.y, ey);
+	ey = fma(w10, e10.y, ey);
+	ey = fma(w11, e11.y, ey);
+}
+
+// Deposit -q/e0 * w at the 4 nodes (order of src/interpolate.c:259-262).
+template <typename T>
+static __device__ __forceinline__ void deposit(const Geo &g, T *sR, double *__restrict__ rho,
+		const CellPos &c, T rx, T ry, T vq)
+{
+	T dxw = (T) 1 - rx, dyw = (T) 1 - ry;
+	T w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
+	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
+	{
+		int i = c.ly * g.W + c.lx;
+		atomicAdd(&sR[i], w00 * vq);
+		atomicAdd(&sR[i + g.W], w01 * vq);
+		atomicAdd(&sR[i + 1], w10 * vq);
+		atomicAdd(&sR[i + g.W + 1], w11 * vq);
+	}
+	else
+	{
+		int i1x = c.ix + 1 == g.nx ? 0 : c.ix + 1;
+		int i1y = c.iy + 1 == g.ny ? 0 : c.iy + 1;
+		unsafeAtomicAdd(&rho[(int64_t) c.iy * g.nx + c.ix], (double) (w00 * vq));
+		unsafeAtomicAdd(&rho[(int64_t) i1y * g.nx + c.ix], (double) (w01 * vq));
+		unsafeAtomicAdd(&rho[(int64_t) c.iy * g.nx + i1x], (double) (w10 * vq));
+		unsafeAtomicAdd(&rho[(int64_t) i1y * g.nx + i1x], (double) (w11 * vq));
+	}
+}
+
+// Cell of a particle and its offsets, both representations.
+static __device__ __forceinline__ void locate(const Geo &g, int ox, int oy, double x, double y,
+		CellPos &c, double &rx, double &ry)
+{
+	cell64(g, x, y, c.ix, c.iy, rx, ry);
+	c.lx = wrapi(c.ix - ox, g.nx);
+	c.ly = wrapi(c.iy - oy, g.ny);
+}
+
+static __device__ __forceinline__ void locate(const Geo &g, int ox, int oy, float x, float y,
+		CellPos &c, float &rx, float &ry)
+{
+	// ox, oy = tile origin - M; stored coordinates are relative to the tile
+	float fx = floorf(x), fy = floorf(y);
+	int cx = (int) fx, cy = (int) fy;
+	rx = x - fx;
+	ry = y - fy;
+	c.lx = cx + g.M;
+	c.ly = cy + g.M;
+	int bx0 = ox + g.M, by0 = oy + g.M;
+	c.ix = (bx0 + cx) & (g.nx - 1);
+	c.iy = (by0 + cy) & (g.ny - 1);
+}
+
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_zero(double *__restrict__ p, int64_t n)
+{
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+			i += (int64_t) gridDim.x * blockDim.x)
+		p[i] = 0.0;
+}
+
+template <typename T, bool IDS, bool PE>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_push(PushArgs a)
+{
+	typedef typename V2<T>::t T2;
+	extern __shared__ unsigned char smem[];
+	__shared__ int s_cnt[2][CPIC_BLOCK / 64];
+	__shared__ int s_kcount;
+	__shared__ double s_red[3][CPIC_BLOCK / 64];
+
+	const Geo &g = a.g;
+	T2 *sE = (T2 *) smem;
+	T *sR = (T *) (smem + sizeof(T2) * g.W * g.H);
+
+	int tile = tile_map(blockIdx.x, g.ntiles);
+	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
+	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
+	int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	unsigned long long lt = (1ull << lane) - 1ull;
+	const T2 *__restrict__ E = (const T2 *) a.E;
+
+	for(int i = tid; i < g.W * g.H; i += CPIC_BLOCK)
+	{
+		int ly = i / g.W, lx = i - ly * g.W;
+		int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
+		if(a.mode != MODE_DEPOSIT)
+			sE[i] = E[(int64_t) gy * g.nx + gx];
+		sR[i] = (T) 0;
+	}
+	if(tid == 0)
+		s_kcount = 0;
+	__syncthreads();
+
+	double ke = 0.0, mpx = 0.0, mpy = 0.0;
+	int buf = 0;
+	bool anybad = false, anynan = false;
+
+	for(int s = 0; s < g.ns; s++)
+	{
+		int b = tile * g.ns + s;
+		int n1 = bucket_n1(a.bk, b);
+		int ntot = n1 + a.bk.n2[b];
+		T h = (T) a.sp.h[s], tz = (T) a.sp.tz[s], sz = (T) a.sp.sz[s];
+		T vq = (T) a.sp.vq[s];
+		double mh = 0.5 * a.sp.m[s];
+		int w = 0;
+
+		for(int base = 0; base < ntot; base += CPIC_BLOCK)
+		{
+			int j = base + tid;
+			bool active = j < ntot;
+			int64_t slot = active ? bucket_slot(a.bk, b, n1, j) : 0;
+			PState<T> q;
+			q.x = q.y = q.ux = q.uy = (T) 0;
+			q.id = 0;
+			q.Ex = q.Ey = 0.0;
+			bool leave = false;
+			int ndst = 0;
+			CellPos c;
+			T rx, ry;
+
+			if(active)
+			{
+				load_p<T>(a.p, slot, q, IDS);
+				locate(g, ox, oy, q.x, q.y, c, rx, ry);
+
+				if(a.mode == MODE_DEPOSIT)
+				{
+					deposit<T>(g, sR, a.rho, c, rx, ry, vq);
+				}
+				else
+				{
+					T ex, ey;
+					gather<T>(g, sE, E, c, rx, ry, ex, ey);
+					q.Ex = (double) ex;
+					q.Ey = (double) ey;
+					if(a.mode == MODE_PUSH)
+					{
+						// boris_rotation reduced to B = (0,0,Bz)
+						T vmx = fma(h, ex, q.ux);
+						T vmy = fma(h, ey, q.uy);
+						T t1 = vmy * tz;
+						T vpx = t1 + vmx;
+						T t2 = vmx * tz;
+						T vpy = vmy - t2;
+						T t3 = vpy * sz;
+						T vplx = t3 + vmx;
+						T t4 = vpx * sz;
+						T vply = vmy - t4;
+						T unx = fma(h, ex, vplx);
+						T uny = fma(h, ey, vply);
+						// check_velocity (src/mover.c:98-137)
+						if(fabs((double) unx) > a.umax_x || fabs((double) uny) > a.umax_y)
+							anybad = true;
+						double ux0 = (double) q.ux, uy0 = (double) q.uy;
+						double ux1 = (double) unx, uy1 = (double) uny;
+						ke += mh * 0.5 * ((ux0 * ux0 + uy0 * uy0) + (ux1 * ux1 + uy1 * uy1));
+						mpx += mh * (ux0 + ux1);
+						mpy += mh * (uy0 + uy1);
+						q.ux = unx;
+						q.uy = uny;
+						if(a.set_r)
+						{
+							if(sizeof(T) == 8)
+							{
+								double nx_ = fma((double) unx, a.dtp, (double) q.x);
+								double ny_ = fma((double) uny, a.dtp, (double) q.y);
+								q.x = (T) wrap64(nx_, g.L0);
+								q.y = (T) wrap64(ny_, g.L1);
+							}
+							else
+							{
+								q.x = (T) fmaf((float) unx, a.dtdx, (float) q.x);
+								q.y = (T) fmaf((float) uny, a.dtdy, (float) q.y);
+							}
+						}
+						if(!(isfinite((double) q.x) && isfinite((double) q.y) &&
+								isfinite((double) q.ux) && isfinite((double) q.uy)))
+						{
+							anynan = true;
+							q.x = q.y = (T) 0;
+						}
+						// new cell, tile; deposit at the new position
+						CellPos c2;
+						locate(g, ox, oy, q.x, q.y, c2, rx, ry);
+						deposit<T>(g, sR, a.rho, c2, rx, ry, vq);
+						int ntix = c2.ix >> g.lgtx, ntiy = c2.iy >> g.lgty;
+						int ntile = ntiy * g.ntx + ntix;
+						if(ntile != tile)
+						{
+							leave = true;
+							ndst = ntile * g.ns + s;
+							if(sizeof(T) == 4)
+							{
+								// re-express relative to the destination tile
+								float fx = floorf((float) q.x), fy = floorf((float) q.y);
+								q.x = (T) ((float) (c2.ix & (g.tx - 1)) + ((float) q.x - fx));
+								q.y = (T) ((float) (c2.iy & (g.ty - 1)) + ((float) q.y - fy));
+							}
+						}
+					}
+				}
+			}
+
+			if(a.mode == MODE_GATHER)
+			{
+				if(active && PE)
+				{
+					a.p.Ex[slot] = q.Ex;
+					a.p.Ey[slot] = q.Ey;
+				}
+				continue;
+			}
+			if(a.mode == MODE_DEPOSIT)
+				continue;
+
+			// leavers: one atomic per wavefront
+			unsigned long long lm = __ballot(leave);
+			int lcnt = __popcll(lm);
+			if(lcnt)
+			{
+				int lbase = 0;
+				if(lane == 0)
+					lbase = atomicAdd(&a.lv.count[a.par], lcnt);
+				lbase = __shfl(lbase, 0);
+				if(leave)
+				{
+					int64_t ls = (int64_t) lbase + __popcll(lm & lt);
+					if(ls < a.lv.cap)
+						store_l<T>(a.lv, ls, q, IDS, PE, ndst, b);
+					else
+					{
+						// no room: stays in this bucket (misplaced);
+						// FP32 coordinates go back to this tile
+						leave = false;
+						if(sizeof(T) == 4)
+						{
+							int dtx = ((ndst / g.ns) & (g.ntx - 1)) * g.tx - (ox + g.M);
+							int dty = ((ndst / g.ns) >> g.lgntx) * g.ty - (oy + g.M);
+							if(dtx > g.nx / 2) dtx -= g.nx;
+							if(dtx < -g.nx / 2) dtx += g.nx;
+							if(dty > g.ny / 2) dty -= g.ny;
+							if(dty < -g.ny / 2) dty += g.ny;
+							q.x = (T) ((float) q.x + (float) dtx);
+							q.y = (T) ((float) q.y + (float) dty);
+						}
+						atomicAdd(&a.ctr->misplaced, 1ull);
+					}
+				}
+				unsigned long long placed = __ballot(leave);
+				if(lane == 0 && placed)
+					atomicAdd(&s_kcount, __popcll(placed));
+			}
+
+			bool stay = active && !leave;
+			unsigned long long sm = __ballot(stay);
+			if(lane == 0)
+				s_cnt[buf][wid] = __popcll(sm);
+			__syncthreads();
+			int pre = 0, tot = 0;
+			for(int k = 0; k < CPIC_BLOCK / 64; k++)
+			{
+				int v = s_cnt[buf][k];
+				pre += k < wid ? v : 0;
+				tot += v;
+			}
+			if(stay)
+			{
+				int dst = w + pre + __popcll(sm & lt);
+				store_p<T>(a.p, a.bk.start[b] + dst, q, IDS, PE);
+			}
+			w += tot;
+			buf ^= 1;
+		}
+
+		if(a.mode == MODE_PUSH)
+		{
+			__syncthreads();
+			if(tid == 0)
+			{
+				a.bk.n1[b] = w;
+				a.bk.res[b] = s_kcount;
+				a.bk.n2[b] = 0;
+				s_kcount = 0;
+			}
+			__syncthreads();
+		}
+	}
+
+	if(anybad)
+		atomicOr(a.derr, CPIC_DERR_UMAX);
+	if(anynan)
+		atomicOr(a.derr, CPIC_DERR_NAN);
+
+	if(a.mode == MODE_GATHER)
+		return;
+
+	__syncthreads();
+	// flush the LDS rho region with FP64 atomics
+	for(int i = tid; i < g.W * g.H; i += CPIC_BLOCK)
+	{
+		T v = sR[i];
+		if(v != (T) 0)
+		{
+			int ly = i / g.W, lx = i - ly * g.W;
+			int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
+			unsafeAtomicAdd(&a.rho[(int64_t) gy * g.nx + gx], (double) v);
+		}
+	}
+
+	if(a.mode == MODE_PUSH && a.diag)
+	{
+		// wave reduction then one atomic per block
+		for(int off = 32; off > 0; off >>= 1)
+		{
+			ke += __shfl_down(ke, off);
+			mpx += __shfl_down(mpx, off);
+			mpy += __shfl_down(mpy, off);
+		}
+		if(lane == 0)
+		{
+			s_red[0][wid] = ke;
+			s_red[1][wid] = mpx;
+			s_red[2][wid] = mpy;
+		}
+		__syncthreads();
+		if(tid == 0)
+		{
+			double k = 0, x = 0, y = 0;
+			for(int i = 0; i < CPIC_BLOCK / 64; i++)
+			{
+				k += s_red[0][i];
+				x += s_red[1][i];
+				y += s_red[2][i];
+			}
+			unsafeAtomicAdd(&a.diag->kinetic, k);
+			unsafeAtomicAdd(&a.diag->px, x);
+			unsafeAtomicAdd(&a.diag->py, y);
+		}
+	}
+}
+
+// Places leavers in their destination bucket, or back in the reserved
+// slots of their source bucket when the destination is full.
+template <typename T, bool IDS, bool PE>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_append(Geo g, Buckets bk, Parts p, Leavers lv, int par, Counters *ctr)
+{
+	int64_t n = lv.count[par];
+	if(n > lv.cap)
+		n = lv.cap;
+	if(blockIdx.x == 0 && threadIdx.x == 0)
+		lv.count[1 - par] = 0;
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+			i += (int64_t) gridDim.x * blockDim.x)
+	{
+		int d = lv.dst[i], s = lv.src[i];
+		PState<T> q;
+		q.x = ((const T *) lv.x)[i];
+		q.y = ((const T *) lv.y)[i];
+		q.ux = ((const T *) lv.ux)[i];
+		q.uy = ((const T *) lv.uy)[i];
+		if(IDS)
+			q.id = lv.id[i];
+		if(PE)
+		{
+			q.Ex = lv.Ex[i];
+			q.Ey = lv.Ey[i];
+		}
+		int slot = atomicAdd(&bk.n1[d], 1);
+		int64_t at;
+		if(slot < bk.cap[d] - bk.res[d])
+			at = bk.start[d] + slot;
+		else
+		{
+			int s2 = atomicAdd(&bk.n2[s], 1);
+			at = bk.start[s] + (bk.cap[s] - bk.res[s]) + s2;
+			if(sizeof(T) == 4)
+			{
+				int td = d / g.ns, ts = s / g.ns;
+				int dtx = ((td & (g.ntx - 1)) - (ts & (g.ntx - 1))) * g.tx;
+				int dty = ((td >> g.lgntx) - (ts >> g.lgntx)) * g.ty;
+				if(dtx > g.nx / 2) dtx -= g.nx;
+				if(dtx < -g.nx / 2) dtx += g.nx;
+				if(dty > g.ny / 2) dty -= g.ny;
+				if(dty < -g.ny / 2) dty += g.ny;
+				q.x = (T) ((float) q.x + (float) dtx);
+				q.y = (T) ((float) q.y + (float) dty);
+			}
+			atomicAdd(&ctr->fallbacks, 1ull);
+		}
+		store_p<T>(p, at, q, IDS, PE);
+	}
+	if(blockIdx.x == 0 && threadIdx.x == 0)
+		atomicAdd(&ctr->leavers, (unsigned long long) n);
+}
+
+// ------------------------------------------------------------- rebuild ---
+
+// Destination bucket of each particle in the current layout (species kept),
+// skipping species flagged in `skip`.
+template <typename T>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_hist_layout(Geo g, Buckets bk, Parts p, int skip, int *__restrict__ cnt)
+{
+	int b = blockIdx.x;
+	int s = b % g.ns, tile = b / g.ns;
+	if(skip & (1 << s))
+		return;
+	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
+	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
+	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
+	for(int j = threadIdx.x; j < ntot; j += blockDim.x)
+	{
+		int64_t slot = bucket_slot(bk, b, n1, j);
+		CellPos c;
+		T rx, ry;
+		locate(g, ox, oy, ((const T *) p.x)[slot], ((const T *) p.y)[slot], c, rx, ry);
+		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
+		atomicAdd(&cnt[nt * g.ns + s], 1);
+	}
+}
+
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_hist_stage(Geo g, int s, int64_t n, const double *__restrict__ x,
+		const double *__restrict__ y, int *__restrict__ cnt)
+{
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+			i += (int64_t) gridDim.x * blockDim.x)
+	{
+		CellPos c;
+		double rx, ry;
+		locate(g, 0, 0, x[i], y[i], c, rx, ry);
+		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
+		atomicAdd(&cnt[nt * g.ns + s], 1);
+	}
+}
+
+// Capacities and exclusive scan of bucket starts (one workgroup).
+__global__ __launch_bounds__(1024)
+void k_scan_caps(const int *__restrict__ cnt, int nb, double slack, int pad, int align,
+		int64_t *__restrict__ start, int *__restrict__ cap, int64_t *__restrict__ total)
+{
+	__shared__ int64_t s[1024];
+	__shared__ int64_t carry;
+	if(threadIdx.x == 0)
+		carry = 0;
+	__syncthreads();
+	for(int base = 0; base < nb; base += 1024)
+	{
+		int b = base + threadIdx.x;
+		int64_t c = 0;
+		if(b < nb)
+		{
+			int64_t n = cnt[b];
+			c = n + (int64_t) ceil((double) n * slack) + pad;
+			c = (c + align - 1) / align * align;
+			if(cap)
+				cap[b] = (int) c;
+		}
+		s[threadIdx.x] = c;
+		__syncthreads();
+		for(int off = 1; off < 1024; off <<= 1)
+		{
+			int64_t v = threadIdx.x >= (unsigned) off ? s[threadIdx.x - off] : 0;
+			__syncthreads();
+			s[threadIdx.x] += v;
+			__syncthreads();
+		}
+		if(b < nb)
+			start[b] = carry + s[threadIdx.x] - c;
+		__syncthreads();
+		if(threadIdx.x == 1023)
+			carry += s[1023];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0)
+		*total = carry;
+}
+
+template <typename T, bool IDS, bool PE>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_scatter_layout(Geo g, Buckets bk, Parts p, int skip, const int64_t *__restrict__ nstart,
+		int *__restrict__ fill, Parts np)
+{
+	int b = blockIdx.x;
+	int s = b % g.ns, tile = b / g.ns;
+	if(skip & (1 << s))
+		return;
+	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
+	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
+	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
+	for(int j = threadIdx.x; j < ntot; j += blockDim.x)
+	{
+		int64_t slot = bucket_slot(bk, b, n1, j);
+		PState<T> q;
+		load_p<T>(p, slot, q, IDS);
+		if(PE)
+		{
+			q.Ex = p.Ex[slot];
+			q.Ey = p.Ey[slot];
+		}
+		CellPos c;
+		T rx, ry;
+		locate(g, ox, oy, q.x, q.y, c, rx, ry);
+		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
+		int nb_ = nt * g.ns + s;
+		if(sizeof(T) == 4)
+		{
+			q.x = (T) ((float) (c.ix & (g.tx - 1)) + (float) rx);
+			q.y = (T) ((float) (c.iy & (g.ty - 1)) + (float) ry);
+		}
+		int k = atomicAdd(&fill[nb_], 1);
+		store_p<T>(np, nstart[nb_] + k, q, IDS, PE);
+	}
+}
+
+template <typename T, bool IDS, bool PE>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_scatter_stage(Geo g, int s, int64_t n, const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ ux,
+		const double *__restrict__ uy, const int64_t *__restrict__ id,
+		const int64_t *__restrict__ nstart, int *__restrict__ fill, Parts np)
+{
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+			i += (int64_t) gridDim.x * blockDim.x)
+	{
+		CellPos c;
+		double rx, ry;
+		locate(g, 0, 0, x[i], y[i], c, rx, ry);
+		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
+		int nb_ = nt * g.ns + s;
+		PState<T> q;
+		if(sizeof(T) == 8)
+		{
+			q.x = (T) x[i];
+			q.y = (T) y[i];
+		}
+		else
+		{
+			q.x = (T) ((float) (c.ix & (g.tx - 1)) + (float) rx);
+			q.y = (T) ((float) (c.iy & (g.ty - 1)) + (float) ry);
+		}
+		q.ux = (T) ux[i];
+		q.uy = (T) uy[i];
+		q.id = IDS ? id[i] : 0;
+		q.Ex = q.Ey = 0.0;
+		int k = atomicAdd(&fill[nb_], 1);
+		store_p<T>(np, nstart[nb_] + k, q, IDS, PE);
+	}
+}
+
+// ------------------------------------------------------------ download ---
+
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_species_counts(Buckets bk, int ns, int s, int ntiles, int *__restrict__ cnt)
+{
+	for(int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+	{
+		int b = t * ns + s;
+		cnt[t] = bucket_n1(bk, b) + bk.n2[b];
+	}
+}
+
+template <typename T, bool IDS, bool PE>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_download(Geo g, Buckets bk, Parts p, int s, const int64_t *__restrict__ off,
+		double *__restrict__ out /* 6 arrays of n */, int64_t *__restrict__ oid,
+		int64_t *__restrict__ ocell, int64_t n)
+{
+	int tile = blockIdx.x;
+	int b = tile * g.ns + s;
+	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
+	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
+	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
+	for(int j = threadIdx.x; j < ntot; j += blockDim.x)
+	{
+		int64_t slot = bucket_slot(bk, b, n1, j);
+		int64_t o = off[tile] + j;
+		T x = ((const T *) p.x)[slot], y = ((const T *) p.y)[slot];
+		CellPos c;
+		T rx, ry;
+		locate(g, ox, oy, x, y, c, rx, ry);
+		double ax, ay;
+		if(sizeof(T) == 8)
+		{
+			ax = (double) x;
+			ay = (double) y;
+		}
+		else
+		{
+			ax = ((double) c.ix + (double) rx) * g.dx;
+			ay = ((double) c.iy + (double) ry) * g.dy;
+		}
+		out[o] = ax;
+		out[n + o] = ay;
+		out[2 * n + o] = (double) ((const T *) p.ux)[slot];
+		out[3 * n + o] = (double) ((const T *) p.uy)[slot];
+		out[4 * n + o] = PE ? p.Ex[slot] : 0.0;
+		out[5 * n + o] = PE ? p.Ey[slot] : 0.0;
+		oid[o] = IDS ? p.id[slot] : -1;
+		ocell[o] = (int64_t) c.iy * g.nx + c.ix;
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__restrict__ counts)
+{
+	int tile = blockIdx.x;
+	int b = tile * g.ns + s;
+	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
+	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
+	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
+	for(int j = threadIdx.x; j < ntot; j += blockDim.x)
+	{
+		int64_t slot = bucket_slot(bk, b, n1, j);
+		CellPos c;
+		T rx, ry;
+		locate(g, ox, oy, ((const T *) p.x)[slot], ((const T *) p.y)[slot], c, rx, ry);
+		atomicAdd(&counts[(int64_t) c.iy * g.nx + c.ix], 1ull);
+	}
+}
+
+// ---------------------------------------------------------------- host ---
+
+#define DISPATCH(kern, T, ...) do { \
+		bool ids_ = (d->desc.flags & CPIC_F_IDS) != 0; \
+		bool pe_ = (d->desc.flags & CPIC_F_PARTICLE_E) != 0; \
+		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true>), __VA_ARGS__); \
+		else if(ids_) hipLaunchKernelGGL((kern<T, true, false>), __VA_ARGS__); \
+		else if(pe_) hipLaunchKernelGGL((kern<T, false, true>), __VA_ARGS__); \
+		else hipLaunchKernelGGL((kern<T, false, false>), __VA_ARGS__); \
+	} while(0)
+
+#define DISPATCH_P(kern, ...) do { \
+		if(d->prec == 64) DISPATCH(kern, double, __VA_ARGS__); \
+		else DISPATCH(kern, float, __VA_ARGS__); \
+	} while(0)
+
+static size_t
+elem(const cpic_dev *d)
+{
+	return d->prec == 64 ? sizeof(double) : sizeof(float);
+}
+
+void
+cpic_parts_free(cpic_dev *d, int which)
+{
+	Parts *p = &d->p[which];
+	hipFree(p->x); hipFree(p->y); hipFree(p->ux); hipFree(p->uy);
+	hipFree(p->id); hipFree(p->Ex); hipFree(p->Ey);
+	memset(p, 0, sizeof(*p));
+	d->pcap[which] = 0;
+}
+
+int
+cpic_parts_alloc(cpic_dev *d, int which, int64_t cap)
+{
+	Parts *p = &d->p[which];
+	size_t e = elem(d);
+	if(cap < 64)
+		cap = 64;
+	cpic_parts_free(d, which);
+	if(hipMalloc(&p->x, e * cap) != hipSuccess || hipMalloc(&p->y, e * cap) != hipSuccess ||
+			hipMalloc(&p->ux, e * cap) != hipSuccess ||
+			hipMalloc(&p->uy, e * cap) != hipSuccess)
+		goto fail;
+	if((d->desc.flags & CPIC_F_IDS) && hipMalloc(&p->id, sizeof(int64_t) * cap) != hipSuccess)
+		goto fail;
+	if((d->desc.flags & CPIC_F_PARTICLE_E) &&
+			(hipMalloc(&p->Ex, sizeof(double) * cap) != hipSuccess ||
+			 hipMalloc(&p->Ey, sizeof(double) * cap) != hipSuccess))
+		goto fail;
+	d->pcap[which] = cap;
+	return 0;
+fail:
+	cpic_parts_free(d, which);
+	return CPIC_ENOMEM;
+}
+
+static PushArgs
+push_args(cpic_dev *d, int64_t iter, int mode, double *rho)
+{
+	PushArgs a;
+	memset(&a, 0, sizeof(a));
+	a.g = d->g;
+	a.bk = d->bk;
+	a.p = d->p[d->pcur];
+	a.lv = d->lv;
+	a.par = d->step_parity;
+	a.E = d->E;
+	a.rho = rho;
+	a.derr = d->derr;
+	a.ctr = d->ctr;
+	a.mode = mode;
+	double dtp = iter == 0 ? -d->dt / 2 : d->dt;
+	a.set_r = iter == 0 ? 0 : 1;
+	a.dtp = dtp;
+	a.dtdx = (float) (dtp / d->g.dx);
+	a.dtdy = (float) (dtp / d->g.dy);
+	a.umax_x = d->umax[0];
+	a.umax_y = d->umax[1];
+	for(int s = 0; s < d->ns; s++)
+	{
+		// dtqm2 = 0.5 * dt * q / m (src/mover.c:211,222); t, s per component
+		double h = 0.5 * dtp * d->desc.q[s] / d->desc.m[s];
+		double tz = d->desc.B[2] * h;
+		double sden = fma(tz, tz, 1.0);
+		a.sp.h[s] = h;
+		a.sp.tz[s] = tz;
+		a.sp.sz[s] = 2.0 * tz / sden;
+		a.sp.vq[s] = -d->desc.q[s] / d->desc.e0;
+		a.sp.m[s] = d->desc.m[s];
+	}
+	if(mode == MODE_PUSH && iter >= 0 && !(d->desc.flags & CPIC_F_NO_DIAG))
+		a.diag = &d->diag[iter % CPIC_DIAG_RING];
+	return a;
+}
+
+static size_t
+push_lds(const cpic_dev *d)
+{
+	size_t n = (size_t) d->g.W * d->g.H;
+	return d->prec == 64 ? n * (sizeof(double2) + sizeof(double))
+		: n * (sizeof(float2) + sizeof(float));
+}
+
+int
+cpic_zero(cpic_dev *d, double *p, int64_t n)
+{
+	int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+	if(blocks > 4096)
+		blocks = 4096;
+	int slot = cpic_kt_begin(d, "zero_rho");
+	hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream, p, n);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int
+cpic_parts_push(cpic_dev *d, int64_t iter)
+{
+	PushArgs a = push_args(d, iter, MODE_PUSH, d->rho[1 - d->rho_cur]);
+	int slot;
+	size_t lds = push_lds(d);
+
+	slot = cpic_kt_begin(d, "push_gather_deposit");
+	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+
+	slot = cpic_kt_begin(d, "append_leavers");
+	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->g, d->bk,
+			d->p[d->pcur], d->lv, d->step_parity, d->ctr);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	d->step_parity ^= 1;
+	return 0;
+}
+
+int
+cpic_parts_deposit(cpic_dev *d, double *rho_out)
+{
+	PushArgs a = push_args(d, 1, MODE_DEPOSIT, rho_out);
+	int slot = cpic_kt_begin(d, "deposit");
+	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), push_lds(d), d->stream, a);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int
+cpic_parts_gather(cpic_dev *d)
+{
+	PushArgs a = push_args(d, 1, MODE_GATHER, d->rho[d->rho_cur]);
+	int slot = cpic_kt_begin(d, "gather");
+	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), push_lds(d), d->stream, a);
+	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// Full re-sort: counts per destination bucket (current layout + pending
+// uploads), fresh capacities, scatter into the spare particle arrays.
+int
+cpic_parts_rebuild(cpic_dev *d)
+{
+	int nb = d->nb, skip = 0, s;
+	int *cnt = NULL, *fill = NULL;
+	int64_t *nstart = NULL, *total = NULL, tot = 0;
+	double *stage = NULL;
+	int64_t *sid = NULL;
+	int ret = CPIC_ENOMEM;
+	int *ncap = NULL;
+
+	for(s = 0; s < d->ns; s++)
+		if(d->up_buf[s])
+			skip |= 1 << s;
+
+	HIPCHK(hipStreamSynchronize(d->stream));
+	if(hipMalloc(&cnt, sizeof(int) * nb) != hipSuccess ||
+			hipMalloc(&fill, sizeof(int) * nb) != hipSuccess ||
+			hipMalloc(&ncap, sizeof(int) * nb) != hipSuccess ||
+			hipMalloc(&nstart, sizeof(int64_t) * nb) != hipSuccess ||
+			hipMalloc(&total, sizeof(int64_t)) != hipSuccess)
+		goto out;
+	hipMemsetAsync(cnt, 0, sizeof(int) * nb, d->stream);
+	hipMemsetAsync(fill, 0, sizeof(int) * nb, d->stream);
+
+	if(d->pcap[d->pcur] > 0)
+	{
+		if(d->prec == 64)
+			hipLaunchKernelGGL(k_hist_layout<double>, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
+					d->g, d->bk, d->p[d->pcur], skip, cnt);
+		else
+			hipLaunchKernelGGL(k_hist_layout<float>, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
+					d->g, d->bk, d->p[d->pcur], skip, cnt);
+	}
+	// stage uploads
+	{
+		int64_t maxn = 0;
+		for(s = 0; s < d->ns; s++)
+			if(d->up_buf[s] && d->up_n[s] > maxn)
+				maxn = d->up_n[s];
+		if(maxn > 0)
+		{
+			if(hipMalloc(&stage, sizeof(double) * 4 * maxn * d->ns) != hipSuccess ||
+					hipMalloc(&sid, sizeof(int64_t) * maxn * d->ns) != hipSuccess)
+				goto out;
+		}
+		for(s = 0; s < d->ns; s++)
+		{
+			int64_t n = d->up_n[s];
+			double *st = stage + (int64_t) 4 * maxn * s;
+			if(!d->up_buf[s] || n == 0)
+				continue;
+			hipMemcpyAsync(st, d->up_buf[s], sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+					d->stream);
+			if(d->up_id[s])
+				hipMemcpyAsync(sid + maxn * s, d->up_id[s], sizeof(int64_t) * n,
+						hipMemcpyHostToDevice, d->stream);
+			int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+			if(blocks > 4096) blocks = 4096;
+			hipLaunchKernelGGL(k_hist_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
+					d->g, s, n, st, st + n, cnt);
+		}
+		// slack > 0: spare fraction; 0: default; < 0: no spare at all (tight
+		// capacities, which exercises the fallback path)
+		double slack = d->desc.slack > 0 ? d->desc.slack : 0.25;
+		int pad = 64;
+		if(d->desc.slack < 0)
+		{
+			slack = 0.0;
+			pad = 0;
+		}
+		hipLaunchKernelGGL(k_scan_caps, dim3(1), dim3(1024), 0, d->stream, cnt, nb, slack, pad,
+				32, nstart, ncap, total);
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipMemcpyAsync(&tot, total, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
+		HIPCHK(hipStreamSynchronize(d->stream));
+
+		int spare = 1 - d->pcur;
+		if(d->pcap[spare] < tot)
+			if((ret = cpic_parts_alloc(d, spare, tot + tot / 8)))
+				goto out;
+		if(d->pcap[d->pcur] > 0)
+			DISPATCH_P(k_scatter_layout, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
+					d->g, d->bk, d->p[d->pcur], skip, nstart, fill, d->p[spare]);
+		for(s = 0; s < d->ns; s++)
+		{
+			int64_t n = d->up_n[s];
+			double *st = stage + (int64_t) 4 * maxn * s;
+			if(!d->up_buf[s] || n == 0)
+				continue;
+			int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+			if(blocks > 4096) blocks = 4096;
+			DISPATCH_P(k_scatter_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
+					d->g, s, n, st, st + n, st + 2 * n, st + 3 * n,
+					d->up_id[s] ? sid + maxn * s : (const int64_t *) NULL, nstart, fill,
+					d->p[spare]);
+		}
+		HIPCHK(hipGetLastError());
+		// new metadata: n1 = count, res = n2 = 0
+		HIPCHK(hipMemcpyAsync(d->bk.start, nstart, sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, d->stream));
+		HIPCHK(hipMemcpyAsync(d->bk.cap, ncap, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
+		HIPCHK(hipMemcpyAsync(d->bk.n1, cnt, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
+		HIPCHK(hipMemsetAsync(d->bk.res, 0, sizeof(int) * nb, d->stream));
+		HIPCHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
+		HIPCHK(hipStreamSynchronize(d->stream));
+		d->pcur = spare;
+	}
+	for(s = 0; s < d->ns; s++)
+	{
+		if(d->up_buf[s])
+		{
+			d->count[s] = d->up_n[s];
+			free(d->up_buf[s]);
+			free(d->up_id[s]);
+			d->up_buf[s] = NULL;
+			d->up_id[s] = NULL;
+			d->up_n[s] = 0;
+		}
+	}
+	d->pending = 0;
+	d->nrebuilds++;
+	ret = 0;
+out:
+	hipFree(cnt); hipFree(fill); hipFree(ncap); hipFree(nstart); hipFree(total);
+	hipFree(stage); hipFree(sid);
+	return ret;
+}
+
+int
+cpic_parts_download(cpic_dev *d, int s, cpic_particles_t *out)
+{
+	int nt = d->g.ntiles;
+	int *cnt = NULL;
+	int64_t *off = NULL, *tot = NULL, n = 0;
+	double *buf = NULL;
+	int64_t *oid = NULL, *ocell = NULL;
+	double *h = NULL;
+	int64_t *hid = NULL, *hcell = NULL;
+	int ret = CPIC_ENOMEM;
+
+	if(hipMalloc(&cnt, sizeof(int) * nt) != hipSuccess ||
+			hipMalloc(&off, sizeof(int64_t) * nt) != hipSuccess ||
+			hipMalloc(&tot, sizeof(int64_t)) != hipSuccess)
+		goto out;
+	hipLaunchKernelGGL(k_species_counts, dim3((nt + CPIC_BLOCK - 1) / CPIC_BLOCK), dim3(CPIC_BLOCK),
+			0, d->stream, d->bk, d->ns, s, nt, cnt);
+	hipLaunchKernelGGL(k_scan_caps, dim3(1), dim3(1024), 0, d->stream, cnt, nt, 0.0, 0, 1,
+			off, (int *) NULL, tot);
+	HIPCHK(hipMemcpyAsync(&n, tot, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
+	HIPCHK(hipStreamSynchronize(d->stream));
+	if(out->n < n && (out->x || out->y || out->ux || out->uy || out->id || out->Ex ||
+				out->Ey || out->cell))
+	{
+		out->n = n;
+		ret = CPIC_EINVAL;
+		goto out;
+	}
+	if(n > 0)
+	{
+		if(hipMalloc(&buf, sizeof(double) * 6 * n) != hipSuccess ||
+				hipMalloc(&oid, sizeof(int64_t) * n) != hipSuccess ||
+				hipMalloc(&ocell, sizeof(int64_t) * n) != hipSuccess)
+			goto out;
+		DISPATCH_P(k_download, dim3(nt), dim3(CPIC_BLOCK), 0, d->stream, d->g, d->bk,
+				d->p[d->pcur], s, off, buf, oid, ocell, n);
+		HIPCHK(hipGetLastError());
+		h = (double *) malloc(sizeof(double) * 6 * n);
+		hid = (int64_t *) malloc(sizeof(int64_t) * n);
+		hcell = (int64_t *) malloc(sizeof(int64_t) * n);
+		if(!h || !hid || !hcell)
+			goto out;
+		HIPCHK(hipMemcpyAsync(h, buf, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, d->stream));
+		HIPCHK(hipMemcpyAsync(hid, oid, sizeof(int64_t) * n, hipMemcpyDeviceToHost, d->stream));
+		HIPCHK(hipMemcpyAsync(hcell, ocell, sizeof(int64_t) * n, hipMemcpyDeviceToHost, d->stream));
+		HIPCHK(hipStreamSynchronize(d->stream));
+		if(out->x) memcpy(out->x, h, sizeof(double) * n);
+		if(out->y) memcpy(out->y, h + n, sizeof(double) * n);
+		if(out->ux) memcpy(out->ux, h + 2 * n, sizeof(double) * n);
+		if(out->uy) memcpy(out->uy, h + 3 * n, sizeof(double) * n);
+		if(out->Ex) memcpy(out->Ex, h + 4 * n, sizeof(double) * n);
+		if(out->Ey) memcpy(out->Ey, h + 5 * n, sizeof(double) * n);
+		if(out->id) memcpy(out->id, hid, sizeof(int64_t) * n);
+		if(out->cell) memcpy(out->cell, hcell, sizeof(int64_t) * n);
+	}
+	out->n = n;
+	ret = 0;
+out:
+	hipFree(cnt); hipFree(off); hipFree(tot); hipFree(buf); hipFree(oid); hipFree(ocell);
+	free(h); free(hid); free(hcell);
+	return ret;
+}
+
+int
+cpic_parts_cell_counts(cpic_dev *d, int s, int64_t *counts)
+{
+	unsigned long long *c = NULL;
+	int64_t n = (int64_t) d->g.nx * d->g.ny;
+	if(hipMalloc(&c, sizeof(unsigned long long) * n) != hipSuccess)
+		return CPIC_ENOMEM;
+	hipMemsetAsync(c, 0, sizeof(unsigned long long) * n, d->stream);
+	if(d->prec == 64)
+		hipLaunchKernelGGL(k_cell_counts<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
+				d->g, d->bk, d->p[d->pcur], s, c);
+	else
+		hipLaunchKernelGGL(k_cell_counts<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
+				d->g, d->bk, d->p[d->pcur], s, c);
+	hipError_t e = hipMemcpyAsync(counts, c, sizeof(int64_t) * n, hipMemcpyDeviceToHost, d->stream);
+	hipStreamSynchronize(d->stream);
+	hipFree(c);
+	return e == hipSuccess ? 0 : CPIC_EHIP;
+}
+
+// Tiles up to 64x64 cells need up to 108 KB of LDS in FP64 mode.
+int
+cpic_parts_init(cpic_dev *d)
+{
+	(void) d;
+	const void *k[] = {
+		(const void *) k_push<double, false, false>, (const void *) k_push<double, true, false>,
+		(const void *) k_push<double, false, true>, (const void *) k_push<double, true, true>,
+		(const void *) k_push<float, false, false>, (const void *) k_push<float, true, false>,
+		(const void *) k_push<float, false, true>, (const void *) k_push<float, true, true>,
+	};
+	for(unsigned i = 0; i < sizeof(k) / sizeof(k[0]); i++)
+		HIPCHK(hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+				CPIC_LDS_LIMIT));
+	return 0;
+}

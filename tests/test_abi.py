@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every symbol include/cpic_hip.h declares
+(no compute calls: these run without a GPU)."""
+import os
+import re
+
+import pytest
+
+import cpic_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "cpic_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cpic_\w+)\s*\(", src)))
+
+
+def test_header_declares_known_entry_points():
+    assert declared() == sorted(ca.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ca.lib()
+    missing = [s for s in declared() if not hasattr(lib, s)]
+    assert missing == []
+
+
+def test_strerror_and_rho_size():
+    lib = ca.lib()
+    for code in ca.ERRORS:
+        assert lib.cpic_dev_strerror(code)
+    n, cnx, cny = ca.solver_rho_size(64, 32)
+    assert (cnx, cny, n) == (66, 32, 66 * 32)
+
+
+def test_invalid_descriptions_rejected_before_any_device_call():
+    bad = [dict(nx=48), dict(B=(0.0, 1.0, 0.0)), dict(precision=16), dict(nchunks=3),
+           dict(tile=(3, 4))]
+    for kw in bad:
+        args = dict(nx=16, ny=16, L=(16.0, 16.0), dt=0.1, q=[-1.0], m=[1.0])
+        args.update(kw)
+        with pytest.raises(ca.CpicError) as ei:
+            ca.Device(**args)
+        assert ei.value.code == -1, kw

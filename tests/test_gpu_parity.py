@@ -1,0 +1,269 @@
+"""HIP path vs the FP64 oracle, through the C ABI (include/cpic_hip.h).
+
+Bar: bit-exact for cell indices, per-cell counts, migration lists and, on a
+shared input state, for the gathered E, pushed velocities and positions;
+1e-12 relative L2 for rho, E, particle state and the field-energy history
+over the first 20 steps (FP64 mode)."""
+import os
+
+import numpy as np
+import pytest
+
+import cpic_amd as ca
+import pyoracle as po
+from common import rel_l2, two_stream
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = ca.F_IDS | ca.F_PARTICLE_E
+
+
+def make_pair(nx, ny, L, dt, q, m, parts, nchunks=1, precision=64, tile=(0, 0), flags=FLAGS,
+              B=(0.0, 0.0, 0.0), e0=1.0, slack=0.0):
+    o = po.Oracle(nx, ny, L, dt, q, m, e0=e0, B=B, nchunks=nchunks)
+    d = ca.Device(nx, ny, L, dt, q, m, e0=e0, B=B, nchunks=nchunks, precision=precision,
+                  tile=tile, flags=flags, slack=slack)
+    for s, (ids, x, y, ux, uy) in enumerate(parts):
+        o.add(s, ids, x, y, ux, uy)
+        d.upload(s, ids, x, y, ux, uy)
+    return o, d
+
+
+def c1(seed=1, n=5120, precision=64, tile=(0, 0), nchunks=1, slack=0.0, flags=FLAGS):
+    """64x64 two-stream: 2 x 5120 electrons, q=-1, m such that w_p = 1."""
+    L = (64.0, 64.0)
+    m = 2 * n / (L[0] * L[1])
+    parts = two_stream(n, L, 0.1, 0.01, seed)
+    return make_pair(64, 64, L, 0.1, [-1.0, -1.0], [m, m], parts, nchunks=nchunks,
+                     precision=precision, tile=tile, slack=slack, flags=flags)
+
+
+def assert_same_particles(o, d, ns, exact=True, tol=1e-12):
+    for s in range(ns):
+        a = o.particles(s)
+        b = d.particles(s)
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("x", "y", "ux", "uy"):
+            if exact:
+                assert np.array_equal(a[k], b[k]), (s, k, np.max(np.abs(a[k] - b[k])))
+            else:
+                assert rel_l2(b[k], a[k]) < tol, (s, k, rel_l2(b[k], a[k]))
+
+
+def test_library_exports_and_device():
+    d = ca.Device(16, 16, (16.0, 16.0), 0.1, [-1.0], [1.0])
+    st = d.stats()
+    assert st["tile"] == [16, 16] and st["precision"] == 64
+    d.close()
+
+
+def test_pre_step_fields_match_oracle():
+    o, d = c1()
+    o.pre_step()
+    d.pre_step()
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert rel_l2(d.field(f), o.field(f)) < 1e-13, f
+
+
+@pytest.mark.parametrize("it", [0, 1])
+def test_push_bitwise_on_shared_state(it):
+    """Same E field and particles: gather, Boris push, wrap are bit-exact."""
+    o, d = c1(seed=2)
+    o.pre_step()
+    d.pre_step()
+    d.set_field("Ex", o.field("Ex"))
+    d.set_field("Ey", o.field("Ey"))
+    o.iter = it
+    o.stage("plasma_E")
+    o.stage("plasma_r")
+    d.stage("plasma_E", it)
+    d.stage("plasma_r", it)
+    for s in range(2):
+        a, b = o.particles(s), d.particles(s)
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("Ex", "Ey", "x", "y", "ux", "uy"):
+            assert np.array_equal(a[k], b[k]), (s, k, np.max(np.abs(a[k] - b[k])))
+
+
+def test_migration_and_cells_bitwise():
+    """Cell indices, per-cell counts and the list of tile leavers (with
+    destinations) equal what the oracle's positions give."""
+    o, d = c1(seed=3, tile=(8, 8))
+    o.pre_step()
+    d.pre_step()
+    d.set_field("Ex", o.field("Ex"))
+    d.set_field("Ey", o.field("Ey"))
+    before = {s: o.particles(s) for s in range(2)}
+    o.iter = 1
+    o.stage("plasma_E")
+    o.stage("plasma_r")
+    d.stage("plasma_E", 1)
+    d.stage("plasma_r", 1)
+    mig = d.migration()
+    d.stage("field_rho", 1)
+    exp_ids, exp_dst = [], []
+    for s in range(2):
+        a = o.particles(s)
+        cells = o.cells(a["x"], a["y"])
+        b = d.particles(s)
+        assert np.array_equal(b["cell"], cells)
+        counts = np.bincount(cells, minlength=64 * 64).reshape(64, 64)
+        assert np.array_equal(d.cell_counts(s), counts)
+        c0 = o.cells(before[s]["x"], before[s]["y"])
+        t0 = (c0 // 64 // 8) * 8 + (c0 % 64) // 8
+        t1 = (cells // 64 // 8) * 8 + (cells % 64) // 8
+        moved = t0 != t1
+        exp_ids += [(s, i) for i in a["id"][moved]]
+        exp_dst += list(zip([s] * moved.sum(), a["id"][moved], t1[moved]))
+    got = sorted(zip(mig["species"].tolist(), mig["id"].tolist(), mig["dst"].tolist()))
+    assert got == sorted((int(s), int(i), int(t)) for s, i, t in exp_dst)
+    assert len(got) > 0
+
+
+def run_both(o, d, nsteps):
+    """Runs pre_step + nsteps on both, returning per-step field energies."""
+    o.pre_step()
+    d.pre_step()
+    W = []
+    for it in range(nsteps):
+        o.stage("field_E")
+        W.append(o.field_energy())
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        o.iter = it + 1
+        d.step(it)
+    return np.array(W)
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (8, 8), (4, 16), (64, 64)])
+def test_20_steps_fp64_rel_l2(tile):
+    o, d = c1(seed=4, tile=tile)
+    W = run_both(o, d, 20)
+    d.sync()
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert rel_l2(d.field(f), o.field(f)) < 1e-12, f
+    assert_same_particles(o, d, 2, exact=False, tol=1e-12)
+    e = d.energy()
+    assert len(e["field"]) == 20
+    assert rel_l2(e["field"], W) < 1e-12
+
+
+def test_20_steps_chunks_and_tight_capacity():
+    """nchunks sets umax only; a capacity with no slack exercises the
+    fallback path (leavers kept in their source tile) without changing
+    results."""
+    o, d = c1(seed=5, tile=(4, 4), nchunks=4, slack=-1.0)
+    run_both(o, d, 20)
+    d.sync()
+    st = d.stats()
+    assert st["leavers"] > 0
+    assert rel_l2(d.field("rho"), o.field("rho")) < 1e-12
+    assert_same_particles(o, d, 2, exact=False, tol=1e-12)
+
+
+def test_rebuild_midway():
+    o, d = c1(seed=6)
+    o.pre_step()
+    d.pre_step()
+    for it in range(10):
+        o.step()
+        d.step(it)
+    d.rebuild()
+    for it in range(10, 20):
+        o.step()
+        d.step(it)
+    assert rel_l2(d.field("rho"), o.field("rho")) < 1e-12
+    assert_same_particles(o, d, 2, exact=False, tol=1e-12)
+
+
+def test_kinetic_energy_diag():
+    o, d = c1(seed=7)
+    o.pre_step()
+    d.pre_step()
+    m = o.params.m[0]
+    K = []
+    for it in range(5):
+        u0 = [o.particles(s) for s in range(2)]
+        o.step()
+        u1 = [o.particles(s) for s in range(2)]
+        K.append(sum(0.25 * m * (np.sum(a["ux"] ** 2 + a["uy"] ** 2) + np.sum(b["ux"] ** 2 + b["uy"] ** 2))
+                     for a, b in zip(u0, u1)))
+        d.step(it)
+    e = d.energy()
+    assert rel_l2(e["kinetic"], K) < 1e-12
+
+
+def test_magnetised_fp64():
+    """Bz with w_c/w_p = 0.5 (Boris rotation active), 32^2, 20 steps."""
+    L = (32.0, 32.0)
+    n = 4096
+    m = 2 * n / (L[0] * L[1])
+    parts = two_stream(n, L, 0.1, 0.01, seed=8)
+    Bz = 0.5 * m          # w_c = |q| B / m
+    o, d = make_pair(32, 32, L, 0.05, [-1.0, -1.0], [m, m], parts, B=(0.0, 0.0, Bz),
+                     tile=(8, 8))
+    run_both(o, d, 20)
+    assert rel_l2(d.field("Ex"), o.field("Ex")) < 1e-12
+    assert_same_particles(o, d, 2, exact=False, tol=1e-12)
+
+
+def test_harmonic_fixture_gpu(golden_dir):
+    """The reference's harm.r0x/harm.E0x (1024^2, 1200 steps) on the GPU."""
+    r0x = np.loadtxt(os.path.join(golden_dir, "harm.r0x"))
+    e0x = np.loadtxt(os.path.join(golden_dir, "harm.E0x"))
+    d = ca.Device(1024, 1024, (8.0, 8.0), 5e-3, [-1.0, -1.0], [1e-2, 1e-2], flags=FLAGS)
+    d.upload(0, [0], [1.0], [4.0], [0.0], [0.0])
+    d.upload(1, [0], [7.0], [4.0], [0.0], [0.0])
+    d.pre_step()
+    rs, es = [], []
+    for it in range(len(r0x)):
+        d.step(it)
+        p = d.particles(0)
+        rs.append(p["x"][0])
+        es.append(p["Ex"][0])
+    np.testing.assert_allclose(rs, r0x, rtol=0, atol=5.1e-7)
+    np.testing.assert_allclose(es, e0x, rtol=6e-7, atol=0)
+
+
+def test_constant_speed_gpu():
+    """test/constant-speed.c: 50,000 steps, velocity constant to 1e-10."""
+    d = ca.Device(64, 64, (32.0, 32.0), 0.5, [-1.0], [1e-2], nchunks=4, flags=FLAGS)
+    d.upload(0, [0], [4.0], [16.0], [1.33333], [0.282828])
+    d.pre_step()
+    d.step(0)
+    u = d.particles(0)
+    ref = (u["ux"][0], u["uy"][0])
+    it = 1
+    for _ in range(50):
+        d.run(it, 1000)
+        it += 1000
+        p = d.particles(0)
+        assert abs(p["ux"][0] - ref[0]) < 1e-10 and abs(p["uy"][0] - ref[1]) < 1e-10
+
+
+def test_fp32_close_to_fp64():
+    o, d = c1(seed=9, precision=32)
+    W = run_both(o, d, 20)
+    d.sync()
+    assert rel_l2(d.field("rho"), o.field("rho")) < 1e-5
+    assert rel_l2(d.field("Ex"), o.field("Ex")) < 1e-4
+    assert_same_particles(o, d, 2, exact=False, tol=1e-5)
+    e = d.energy()
+    assert rel_l2(e["field"], W) < 1e-4
+
+
+def test_umax_violation_reported():
+    d = ca.Device(16, 16, (16.0, 16.0), 1.0, [-1.0], [1.0], nchunks=4)
+    d.upload(0, [0], [1.0], [1.0], [4.5], [0.0])
+    d.pre_step()
+    d.step(0)
+    with pytest.raises(ca.CpicError) as ei:
+        d.sync()
+    assert ei.value.code == -3
+
+
+def test_in_plane_B_rejected():
+    with pytest.raises(ca.CpicError) as ei:
+        ca.Device(16, 16, (16.0, 16.0), 0.1, [-1.0], [1.0], B=(0.0, 1.0, 0.0))
+    assert ei.value.code == -1
