@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Throughput of the full 2D electrostatic PIC step on one MI355X.
+
+Workload (mixed-precision two-stream at scale): periodic 2048 x 2048 grid,
+unit cells, 268,435,456 particles (64 per cell) in two species -- electrons
+(q=-1, m=32, i.e. w_p = 1) counter-streaming at +-0.1 with a 0.01 Gaussian
+spread, and mobile ions (q=+1, m=1836*32, spread 0.01/sqrt(1836)) -- B = 0,
+dt = 0.05.  One step = FFT Poisson solve (FP64) -> E -> CIC gather -> Boris
+push -> periodic wrap + tile migration -> CIC deposit.  Particles are FP32
+(tile-relative positions); the same workload in FP64 is timed for
+comparison.  Inputs are generated on the host (synthetic, seeded), uploaded,
+and resident in HBM before the timed region.
+
+Prints one JSON line.  `value` = particle-steps/s of the FP32 step.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "particle-steps/s per full PIC step at 2048^2, 64 ppc; FP32 vs FP64 GPU; %HBM"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen_species(rng, n, L, v0, sigma):
+    x = rng.uniform(0.0, L, n)
+    y = rng.uniform(0.0, L, n)
+    ux = rng.normal(0.0, sigma, n)
+    if v0:
+        ux[0::2] += v0
+        ux[1::2] -= v0
+    uy = rng.normal(0.0, sigma, n)
+    return x, y, ux, uy
+
+
+def workload(grid, ppc):
+    n_cells = grid * grid
+    n_e = n_cells * ppc // 2
+    n_i = n_cells * ppc - n_e
+    m_e = n_e / n_cells          # w_p = 1 with q = -1, e0 = 1, unit cells
+    return dict(grid=grid, L=float(grid), n=(n_e, n_i), q=(-1.0, 1.0),
+                m=(m_e, 1836.0 * m_e), v0=(0.1, 0.0),
+                sigma=(0.01, 0.01 / np.sqrt(1836.0)), dt=0.05)
+
+
+def make_device(ca, w, precision, flags, seed=2024):
+    d = ca.Device(w["grid"], w["grid"], (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]),
+                  nchunks=1, precision=precision, flags=flags)
+    for s in range(2):
+        rng = np.random.default_rng([seed, s])
+        x, y, ux, uy = gen_species(rng, w["n"][s], w["L"], w["v0"][s], w["sigma"][s])
+        d.upload(s, np.empty(0, np.int64), x, y, ux, uy)
+        del x, y, ux, uy
+    return d
+
+
+def time_steps(d, it0, warmup, steps):
+    d.pre_step()
+    d.run(it0, warmup)
+    d.sync()
+    d.reset_kernel_times()
+    t0 = time.perf_counter()
+    d.run(it0 + warmup, steps)
+    d.sync()
+    t1 = time.perf_counter()
+    return (t1 - t0), d.kernel_times()
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(w, steps=10):
+    """The FP64 oracle (C restatement of the reference loop, OpenMP over 64
+    chunks) on the full grid with 1/64 of the particles; the step time of the
+    full workload is extrapolated as t_fields + 64 * t_particles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    frac = 64
+    grid = w["grid"]
+    o = po.Oracle(grid, grid, (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]), nchunks=64)
+    nsample = 0
+    for s in range(2):
+        n = w["n"][s] // frac
+        rng = np.random.default_rng([7, s])
+        x, y, ux, uy = gen_species(rng, n, w["L"], w["v0"][s], w["sigma"][s])
+        o.add(s, np.arange(n, dtype=np.int64), x, y, ux, uy)
+        nsample += n
+    o.pre_step()
+    o.step()                                  # iteration 0
+    tf = tp = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.stage("field_E")
+        t1 = time.perf_counter()
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        t2 = time.perf_counter()
+        o.iter = o.iter + 1
+        tf += t1 - t0
+        tp += t2 - t1
+    tf /= steps
+    tp /= steps
+    t_full = tf + frac * tp
+    ntot = sum(w["n"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": ntot / t_full,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle FP64 on {grid}^2 with {nsample} particles (1/{frac} of the workload), "
+                   f"{steps} steps: fields {tf * 1e3:.1f} ms/step, particles {tp * 1e3:.1f} "
+                   f"ms/step ({tp / nsample * 1e9:.1f} ns/particle-step); full step extrapolated "
+                   f"to {t_full:.2f} s; CPU: {cpu_model()}, {threads} threads, radix-2 FFT"),
+    }
+
+
+def traffic_from_profiles(precision):
+    """HBM bytes per push launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if present."""
+    path = os.path.join(ROOT, "profiles", f"pmc_push_fp{precision}_2048.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=", 1)[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--ppc", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=50, help="0 skips the FP64 comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cpic_amd as ca
+    w = workload(args.grid, args.ppc)
+    ntot = sum(w["n"])
+    ng = args.grid * args.grid
+
+    res = {}
+    for prec, steps in ((32, args.steps), (64, args.fp64_steps)):
+        if steps <= 0:
+            continue
+        t0 = time.time()
+        d = make_device(ca, w, prec, ca.F_PROFILE)
+        log(f"fp{prec}: init {time.time() - t0:.1f} s")
+        el, kt = time_steps(d, 0, args.warmup, steps)
+        st = d.stats()
+        log(f"fp{prec}: {steps} steps in {el:.3f} s, {el / steps * 1e3:.3f} ms/step, {st}")
+        for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
+            log(f"   {k:24s} {n:6d} launches {ms / max(n, 1):9.4f} ms avg")
+        res[prec] = dict(elapsed=el, steps=steps, kt=kt, stats=st)
+        d.close()
+        del d
+
+    r32 = res[32]
+    value = ntot * r32["steps"] / r32["elapsed"]
+    push_n, push_ms = r32["kt"].get("push_gather_deposit", (0, 0.0))
+    avg_push = push_ms / max(push_n, 1) / 1e3
+    alg_bytes = ntot * 32 + ng * 24
+    achieved = alg_bytes / avg_push / 1e9 if avg_push > 0 else None
+    traffic = traffic_from_profiles(32)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["elapsed"] / r32["steps"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded two-stream electrons + mobile ions, host-generated)",
+        "config": {"workload": f"two-stream {args.grid}^2, {args.ppc} ppc, {ntot} particles, "
+                               f"electrons + ions (1:1836), FP32 particles + FP64 FFT Poisson",
+                   "grid": [args.grid, args.grid], "particles": ntot, "ppc": args.ppc,
+                   "dt": w["dt"], "tile": r32["stats"]["tile"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                     "traffic": traffic,
+                     "kernel": "push_gather_deposit",
+                     "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": avg_push * 1e3},
+        "kernels_ms": {k: ms / max(n, 1) for k, (n, ms) in r32["kt"].items()},
+    }
+    if 64 in res:
+        r64 = res[64]
+        v64 = ntot * r64["steps"] / r64["elapsed"]
+        n64, ms64 = r64["kt"].get("push_gather_deposit", (0, 0.0))
+        a64 = ms64 / max(n64, 1) / 1e3
+        out["fp64"] = {"value": v64, "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
+                       "push_achieved_GBs": (ntot * 64 + ng * 24) / a64 / 1e9 if a64 else None}
+        out["fp32_over_fp64"] = value / v64
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w)
+        except Exception as e:  # the baseline must not hide the GPU result
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
