@@ -55,9 +55,9 @@ def workload(grid, ppc):
                 sigma=(0.01, 0.01 / np.sqrt(1836.0)), dt=0.05)
 
 
-def make_device(ca, w, precision, flags, seed=2024):
+def make_device(ca, w, precision, flags, seed=2024, tile=(0, 0)):
     d = ca.Device(w["grid"], w["grid"], (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]),
-                  nchunks=1, precision=precision, flags=flags)
+                  nchunks=1, precision=precision, flags=flags, tile=tile)
     for s in range(2):
         rng = np.random.default_rng([seed, s])
         x, y, ux, uy = gen_species(rng, w["n"][s], w["L"], w["v0"][s], w["sigma"][s])
@@ -151,6 +151,23 @@ def traffic_from_profiles(precision):
     return None
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Plain device-to-device copies of a known size (the runtime's copy
+    kernel), used to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} x {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +177,9 @@ def main():
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=50, help="0 skips the FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 1 GiB device-to-device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -169,6 +189,8 @@ def main():
         sys.exit(2)
 
     import cpic_amd as ca
+    if args.calibrate_copy:
+        copy_calibration()
     w = workload(args.grid, args.ppc)
     ntot = sum(w["n"])
     ng = args.grid * args.grid
@@ -178,7 +200,7 @@ def main():
         if steps <= 0:
             continue
         t0 = time.time()
-        d = make_device(ca, w, prec, ca.F_PROFILE)
+        d = make_device(ca, w, prec, ca.F_PROFILE, tile=(args.tile, args.tile))
         log(f"fp{prec}: init {time.time() - t0:.1f} s")
         el, kt = time_steps(d, 0, args.warmup, steps)
         st = d.stats()

@@ -7,8 +7,10 @@
 //   applies the Boris push        (boris_rotation/move, src/mover.c:23-83, 141-227)
 //   wraps periodically            (periodic_boundary_ppack, src/comm_plasma.c:726-747)
 //   deposits -q/e0 * w into an LDS rho region (interpolate_p2f_rho,
-//                                  src/interpolate.c:283-346; FP64 LDS atomics in
-//                                  FP64 mode, FP32 LDS atomics in FP32 mode)
+//                                  src/interpolate.c:283-346) with FP64 LDS atomics
+//                                  in both modes: on gfx950 ds_add_f64 is cheap while
+//                                  ds_add_f32 costs more than the whole HBM stream
+//                                  (measured at 2048^2: 5.8 ms vs 2.1 ms per launch)
 //   and compacts the tile's stayers in place while leavers (particles whose
 //   cell moved to another tile: comm_plasma's collect, src/comm_plasma.c:
 //   101-165) are appended to a leaver buffer through one wave-aggregated
@@ -232,7 +234,7 @@ static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>
 
 // Deposit -q/e0 * w at the 4 nodes (order of src/interpolate.c:259-262).
 template <typename T>
-static __device__ __forceinline__ void deposit(const Geo &g, T *sR, double *__restrict__ rho,
+static __device__ __forceinline__ void deposit(const Geo &g, double *sR, double *__restrict__ rho,
 		const CellPos &c, T rx, T ry, T vq)
 {
 	T dxw = (T) 1 - rx, dyw = (T) 1 - ry;
@@ -240,10 +242,10 @@ static __device__ __forceinline__ void deposit(const Geo &g, T *sR, double *__re
 	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
 	{
 		int i = c.ly * g.W + c.lx;
-		atomicAdd(&sR[i], w00 * vq);
-		atomicAdd(&sR[i + g.W], w01 * vq);
-		atomicAdd(&sR[i + 1], w10 * vq);
-		atomicAdd(&sR[i + g.W + 1], w11 * vq);
+		atomicAdd(&sR[i], (double) (w00 * vq));
+		atomicAdd(&sR[i + g.W], (double) (w01 * vq));
+		atomicAdd(&sR[i + 1], (double) (w10 * vq));
+		atomicAdd(&sR[i + g.W + 1], (double) (w11 * vq));
 	}
 	else
 	{
@@ -288,19 +290,135 @@ void k_zero(double *__restrict__ p, int64_t n)
 		p[i] = 0.0;
 }
 
-template <typename T, bool IDS, bool PE>
+// Loads the tile's E region into LDS (periodic indexing) and zeroes the
+// LDS rho region.
+template <typename T>
+static __device__ __forceinline__ void tile_setup(const Geo &g, int ox, int oy,
+		const typename V2<T>::t *__restrict__ E, typename V2<T>::t *sE, double *sR)
+{
+	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	{
+		int ly = i / g.W, lx = i - ly * g.W;
+		int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
+		if(sE)
+			sE[i] = E[(int64_t) gy * g.nx + gx];
+		if(sR)
+			sR[i] = 0.0;
+	}
+}
+
+// Adds the LDS rho region into the global rho with FP64 atomics.
+template <typename T>
+static __device__ __forceinline__ void tile_flush(const Geo &g, int ox, int oy, const double *sR,
+		double *__restrict__ rho)
+{
+	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	{
+		double v = sR[i];
+		if(v != 0.0)
+		{
+			int ly = i / g.W, lx = i - ly * g.W;
+			int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
+			unsafeAtomicAdd(&rho[(int64_t) gy * g.nx + gx], (double) v);
+		}
+	}
+}
+
+// One particle: gather, Boris push, move + wrap, deposit at the new
+// position; returns whether its cell left the tile (and the destination).
+template <typename T>
+static __device__ __forceinline__ bool push_one(const PushArgs &a, const Geo &g,
+		const typename V2<T>::t *sE, const typename V2<T>::t *__restrict__ E, double *sR,
+		int tile, int ox, int oy, int s, T h, T tz, T sz, T vq, double mh,
+		PState<T> &q, int &ndst, double &ke, double &mpx, double &mpy, bool &bad, bool &nan_)
+{
+	CellPos c;
+	T rx, ry, ex, ey;
+	locate(g, ox, oy, q.x, q.y, c, rx, ry);
+	gather<T>(g, sE, E, c, rx, ry, ex, ey);
+	q.Ex = (double) ex;
+	q.Ey = (double) ey;
+	// boris_rotation reduced to B = (0,0,Bz)
+	T vmx = fma(h, ex, q.ux);
+	T vmy = fma(h, ey, q.uy);
+	T t1 = vmy * tz;
+	T vpx = t1 + vmx;
+	T t2 = vmx * tz;
+	T vpy = vmy - t2;
+	T t3 = vpy * sz;
+	T vplx = t3 + vmx;
+	T t4 = vpx * sz;
+	T vply = vmy - t4;
+	T unx = fma(h, ex, vplx);
+	T uny = fma(h, ey, vply);
+	// check_velocity (src/mover.c:98-137)
+	if(fabs((double) unx) > a.umax_x || fabs((double) uny) > a.umax_y)
+		bad = true;
+	double ux0 = (double) q.ux, uy0 = (double) q.uy;
+	double ux1 = (double) unx, uy1 = (double) uny;
+	ke += mh * 0.5 * ((ux0 * ux0 + uy0 * uy0) + (ux1 * ux1 + uy1 * uy1));
+	mpx += mh * (ux0 + ux1);
+	mpy += mh * (uy0 + uy1);
+	q.ux = unx;
+	q.uy = uny;
+	if(a.set_r)
+	{
+		if(sizeof(T) == 8)
+		{
+			double nx_ = fma((double) unx, a.dtp, (double) q.x);
+			double ny_ = fma((double) uny, a.dtp, (double) q.y);
+			q.x = (T) wrap64(nx_, g.L0);
+			q.y = (T) wrap64(ny_, g.L1);
+		}
+		else
+		{
+			q.x = (T) fmaf((float) unx, a.dtdx, (float) q.x);
+			q.y = (T) fmaf((float) uny, a.dtdy, (float) q.y);
+		}
+	}
+	if(!(isfinite((double) q.x) && isfinite((double) q.y) &&
+			isfinite((double) q.ux) && isfinite((double) q.uy)))
+	{
+		nan_ = true;
+		q.x = q.y = (T) 0;
+	}
+	CellPos c2;
+	locate(g, ox, oy, q.x, q.y, c2, rx, ry);
+	deposit<T>(g, sR, a.rho, c2, rx, ry, vq);
+	int ntile = (c2.iy >> g.lgty) * g.ntx + (c2.ix >> g.lgtx);
+	if(ntile == tile)
+		return false;
+	ndst = ntile * g.ns + s;
+	if(sizeof(T) == 4)
+	{
+		// re-express relative to the destination tile
+		float fx = floorf((float) q.x), fy = floorf((float) q.y);
+		q.x = (T) ((float) (c2.ix & (g.tx - 1)) + ((float) q.x - fx));
+		q.y = (T) ((float) (c2.iy & (g.ty - 1)) + ((float) q.y - fy));
+	}
+	return true;
+}
+
+// Fused gather + push + wrap + deposit + in-place tile compaction.  Each
+// thread handles U particles per chunk (particle base + u*256 + tid), all
+// loads of a chunk issued before any of them is used, so a workgroup keeps
+// U*256 particles of HBM reads in flight.  Stayers are written in their
+// original order at the front of the bucket (their write index never
+// exceeds a read index of the same or a later chunk); leavers go to the
+// leaver buffer with one atomic per wavefront.
+template <typename T, bool IDS, bool PE, int U>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
 	extern __shared__ unsigned char smem[];
-	__shared__ int s_cnt[2][CPIC_BLOCK / 64];
+	__shared__ int s_cnt[2][U][CPIC_BLOCK / 64];
 	__shared__ int s_kcount;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
 
 	const Geo &g = a.g;
 	T2 *sE = (T2 *) smem;
-	T *sR = (T *) (smem + sizeof(T2) * g.W * g.H);
+	double *sR = (double *) (smem + sizeof(T2) * g.W * g.H);
 
 	int tile = tile_map(blockIdx.x, g.ntiles);
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
@@ -309,238 +427,143 @@ void k_push(PushArgs a)
 	unsigned long long lt = (1ull << lane) - 1ull;
 	const T2 *__restrict__ E = (const T2 *) a.E;
 
-	for(int i = tid; i < g.W * g.H; i += CPIC_BLOCK)
-	{
-		int ly = i / g.W, lx = i - ly * g.W;
-		int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
-		if(a.mode != MODE_DEPOSIT)
-			sE[i] = E[(int64_t) gy * g.nx + gx];
-		sR[i] = (T) 0;
-	}
+	tile_setup<T>(g, ox, oy, E, sE, sR);
 	if(tid == 0)
 		s_kcount = 0;
 	__syncthreads();
 
 	double ke = 0.0, mpx = 0.0, mpy = 0.0;
 	int buf = 0;
-	bool anybad = false, anynan = false;
+	bool bad = false, nan_ = false;
 
 	for(int s = 0; s < g.ns; s++)
 	{
-		int b = tile * g.ns + s;
-		int n1 = bucket_n1(a.bk, b);
-		int ntot = n1 + a.bk.n2[b];
-		T h = (T) a.sp.h[s], tz = (T) a.sp.tz[s], sz = (T) a.sp.sz[s];
-		T vq = (T) a.sp.vq[s];
-		double mh = 0.5 * a.sp.m[s];
+		const int b = tile * g.ns + s;
+		const int64_t bstart = a.bk.start[b];
+		const int cap = a.bk.cap[b], res = a.bk.res[b];
+		const int n1 = min(a.bk.n1[b], cap - res);
+		const int ntot = n1 + a.bk.n2[b];
+		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
+		const T h = (T) a.sp.h[s], tz = (T) a.sp.tz[s], sz = (T) a.sp.sz[s];
+		const T vq = (T) a.sp.vq[s];
+		const double mh = 0.5 * a.sp.m[s];
 		int w = 0;
 
-		for(int base = 0; base < ntot; base += CPIC_BLOCK)
+		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
 		{
-			int j = base + tid;
-			bool active = j < ntot;
-			int64_t slot = active ? bucket_slot(a.bk, b, n1, j) : 0;
-			PState<T> q;
-			q.x = q.y = q.ux = q.uy = (T) 0;
-			q.id = 0;
-			q.Ex = q.Ey = 0.0;
-			bool leave = false;
-			int ndst = 0;
-			CellPos c;
-			T rx, ry;
-
-			if(active)
+			PState<T> q[U];
+			bool act[U], leave[U];
+			int ndst[U];
+#pragma unroll
+			for(int u = 0; u < U; u++)
 			{
-				load_p<T>(a.p, slot, q, IDS);
-				locate(g, ox, oy, q.x, q.y, c, rx, ry);
-
-				if(a.mode == MODE_DEPOSIT)
+				int j = base + u * CPIC_BLOCK + tid;
+				act[u] = j < ntot;
+				int64_t slot = bstart + j + (j >= n1 ? seg2 : 0);
+				if(act[u])
+					load_p<T>(a.p, slot, q[u], IDS);
+			}
+#pragma unroll
+			for(int u = 0; u < U; u++)
+			{
+				leave[u] = false;
+				ndst[u] = 0;
+				if(act[u])
+					leave[u] = push_one<T>(a, g, sE, E, sR, tile, ox, oy, s, h, tz, sz, vq, mh,
+							q[u], ndst[u], ke, mpx, mpy, bad, nan_);
+			}
+			// leavers: one atomic per wavefront
+#pragma unroll
+			for(int u = 0; u < U; u++)
+			{
+				unsigned long long lm = __ballot(leave[u]);
+				if(lm)
 				{
-					deposit<T>(g, sR, a.rho, c, rx, ry, vq);
-				}
-				else
-				{
-					T ex, ey;
-					gather<T>(g, sE, E, c, rx, ry, ex, ey);
-					q.Ex = (double) ex;
-					q.Ey = (double) ey;
-					if(a.mode == MODE_PUSH)
+					int lcnt = __popcll(lm);
+					int lbase = 0;
+					if(lane == 0)
+						lbase = atomicAdd(&a.lv.count[a.par], lcnt);
+					lbase = __shfl(lbase, 0);
+					if(leave[u])
 					{
-						// boris_rotation reduced to B = (0,0,Bz)
-						T vmx = fma(h, ex, q.ux);
-						T vmy = fma(h, ey, q.uy);
-						T t1 = vmy * tz;
-						T vpx = t1 + vmx;
-						T t2 = vmx * tz;
-						T vpy = vmy - t2;
-						T t3 = vpy * sz;
-						T vplx = t3 + vmx;
-						T t4 = vpx * sz;
-						T vply = vmy - t4;
-						T unx = fma(h, ex, vplx);
-						T uny = fma(h, ey, vply);
-						// check_velocity (src/mover.c:98-137)
-						if(fabs((double) unx) > a.umax_x || fabs((double) uny) > a.umax_y)
-							anybad = true;
-						double ux0 = (double) q.ux, uy0 = (double) q.uy;
-						double ux1 = (double) unx, uy1 = (double) uny;
-						ke += mh * 0.5 * ((ux0 * ux0 + uy0 * uy0) + (ux1 * ux1 + uy1 * uy1));
-						mpx += mh * (ux0 + ux1);
-						mpy += mh * (uy0 + uy1);
-						q.ux = unx;
-						q.uy = uny;
-						if(a.set_r)
+						int64_t ls = (int64_t) lbase + __popcll(lm & lt);
+						if(ls < a.lv.cap)
+							store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
+						else
 						{
-							if(sizeof(T) == 8)
-							{
-								double nx_ = fma((double) unx, a.dtp, (double) q.x);
-								double ny_ = fma((double) uny, a.dtp, (double) q.y);
-								q.x = (T) wrap64(nx_, g.L0);
-								q.y = (T) wrap64(ny_, g.L1);
-							}
-							else
-							{
-								q.x = (T) fmaf((float) unx, a.dtdx, (float) q.x);
-								q.y = (T) fmaf((float) uny, a.dtdy, (float) q.y);
-							}
-						}
-						if(!(isfinite((double) q.x) && isfinite((double) q.y) &&
-								isfinite((double) q.ux) && isfinite((double) q.uy)))
-						{
-							anynan = true;
-							q.x = q.y = (T) 0;
-						}
-						// new cell, tile; deposit at the new position
-						CellPos c2;
-						locate(g, ox, oy, q.x, q.y, c2, rx, ry);
-						deposit<T>(g, sR, a.rho, c2, rx, ry, vq);
-						int ntix = c2.ix >> g.lgtx, ntiy = c2.iy >> g.lgty;
-						int ntile = ntiy * g.ntx + ntix;
-						if(ntile != tile)
-						{
-							leave = true;
-							ndst = ntile * g.ns + s;
+							// no room: stays in this bucket (misplaced);
+							// FP32 coordinates go back to this tile
+							leave[u] = false;
 							if(sizeof(T) == 4)
 							{
-								// re-express relative to the destination tile
-								float fx = floorf((float) q.x), fy = floorf((float) q.y);
-								q.x = (T) ((float) (c2.ix & (g.tx - 1)) + ((float) q.x - fx));
-								q.y = (T) ((float) (c2.iy & (g.ty - 1)) + ((float) q.y - fy));
+								int dt_ = ndst[u] / g.ns;
+								int dtx = (dt_ & (g.ntx - 1)) * g.tx - (ox + g.M);
+								int dty = (dt_ >> g.lgntx) * g.ty - (oy + g.M);
+								if(dtx > g.nx / 2) dtx -= g.nx;
+								if(dtx < -g.nx / 2) dtx += g.nx;
+								if(dty > g.ny / 2) dty -= g.ny;
+								if(dty < -g.ny / 2) dty += g.ny;
+								q[u].x = (T) ((float) q[u].x + (float) dtx);
+								q[u].y = (T) ((float) q[u].y + (float) dty);
 							}
+							atomicAdd(&a.ctr->misplaced, 1ull);
 						}
 					}
+					unsigned long long placed = __ballot(leave[u]);
+					if(lane == 0 && placed)
+						atomicAdd(&s_kcount, __popcll(placed));
 				}
 			}
-
-			if(a.mode == MODE_GATHER)
+			// stayers: block-wide stable compaction
+			unsigned long long sm[U];
+#pragma unroll
+			for(int u = 0; u < U; u++)
 			{
-				if(active && PE)
-				{
-					a.p.Ex[slot] = q.Ex;
-					a.p.Ey[slot] = q.Ey;
-				}
-				continue;
-			}
-			if(a.mode == MODE_DEPOSIT)
-				continue;
-
-			// leavers: one atomic per wavefront
-			unsigned long long lm = __ballot(leave);
-			int lcnt = __popcll(lm);
-			if(lcnt)
-			{
-				int lbase = 0;
+				sm[u] = __ballot(act[u] && !leave[u]);
 				if(lane == 0)
-					lbase = atomicAdd(&a.lv.count[a.par], lcnt);
-				lbase = __shfl(lbase, 0);
-				if(leave)
-				{
-					int64_t ls = (int64_t) lbase + __popcll(lm & lt);
-					if(ls < a.lv.cap)
-						store_l<T>(a.lv, ls, q, IDS, PE, ndst, b);
-					else
-					{
-						// no room: stays in this bucket (misplaced);
-						// FP32 coordinates go back to this tile
-						leave = false;
-						if(sizeof(T) == 4)
-						{
-							int dtx = ((ndst / g.ns) & (g.ntx - 1)) * g.tx - (ox + g.M);
-							int dty = ((ndst / g.ns) >> g.lgntx) * g.ty - (oy + g.M);
-							if(dtx > g.nx / 2) dtx -= g.nx;
-							if(dtx < -g.nx / 2) dtx += g.nx;
-							if(dty > g.ny / 2) dty -= g.ny;
-							if(dty < -g.ny / 2) dty += g.ny;
-							q.x = (T) ((float) q.x + (float) dtx);
-							q.y = (T) ((float) q.y + (float) dty);
-						}
-						atomicAdd(&a.ctr->misplaced, 1ull);
-					}
-				}
-				unsigned long long placed = __ballot(leave);
-				if(lane == 0 && placed)
-					atomicAdd(&s_kcount, __popcll(placed));
+					s_cnt[buf][u][wid] = __popcll(sm[u]);
 			}
-
-			bool stay = active && !leave;
-			unsigned long long sm = __ballot(stay);
-			if(lane == 0)
-				s_cnt[buf][wid] = __popcll(sm);
 			__syncthreads();
-			int pre = 0, tot = 0;
-			for(int k = 0; k < CPIC_BLOCK / 64; k++)
+			int run = w;
+#pragma unroll
+			for(int u = 0; u < U; u++)
 			{
-				int v = s_cnt[buf][k];
-				pre += k < wid ? v : 0;
-				tot += v;
+				int pre = 0, tot = 0;
+#pragma unroll
+				for(int k = 0; k < CPIC_BLOCK / 64; k++)
+				{
+					int v = s_cnt[buf][u][k];
+					pre += k < wid ? v : 0;
+					tot += v;
+				}
+				if(act[u] && !leave[u])
+					store_p<T>(a.p, bstart + run + pre + __popcll(sm[u] & lt), q[u], IDS, PE);
+				run += tot;
 			}
-			if(stay)
-			{
-				int dst = w + pre + __popcll(sm & lt);
-				store_p<T>(a.p, a.bk.start[b] + dst, q, IDS, PE);
-			}
-			w += tot;
+			w = run;
 			buf ^= 1;
 		}
 
-		if(a.mode == MODE_PUSH)
+		__syncthreads();
+		if(tid == 0)
 		{
-			__syncthreads();
-			if(tid == 0)
-			{
-				a.bk.n1[b] = w;
-				a.bk.res[b] = s_kcount;
-				a.bk.n2[b] = 0;
-				s_kcount = 0;
-			}
-			__syncthreads();
+			a.bk.n1[b] = w;
+			a.bk.res[b] = s_kcount;
+			a.bk.n2[b] = 0;
+			s_kcount = 0;
 		}
+		__syncthreads();
 	}
 
-	if(anybad)
+	if(bad)
 		atomicOr(a.derr, CPIC_DERR_UMAX);
-	if(anynan)
+	if(nan_)
 		atomicOr(a.derr, CPIC_DERR_NAN);
 
-	if(a.mode == MODE_GATHER)
-		return;
+	tile_flush<T>(g, ox, oy, sR, a.rho);
 
-	__syncthreads();
-	// flush the LDS rho region with FP64 atomics
-	for(int i = tid; i < g.W * g.H; i += CPIC_BLOCK)
+	if(a.diag)
 	{
-		T v = sR[i];
-		if(v != (T) 0)
-		{
-			int ly = i / g.W, lx = i - ly * g.W;
-			int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
-			unsafeAtomicAdd(&a.rho[(int64_t) gy * g.nx + gx], (double) v);
-		}
-	}
-
-	if(a.mode == MODE_PUSH && a.diag)
-	{
-		// wave reduction then one atomic per block
 		for(int off = 32; off > 0; off >>= 1)
 		{
 			ke += __shfl_down(ke, off);
@@ -569,6 +592,72 @@ void k_push(PushArgs a)
 		}
 	}
 }
+
+// stage_field_rho from stored positions: rho_reset + interpolate_p2f_rho
+// (src/field.c:164-230, src/interpolate.c:283-346).
+template <typename T>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_deposit(PushArgs a)
+{
+	extern __shared__ unsigned char smem[];
+	const Geo &g = a.g;
+	double *sR = (double *) smem;
+	int tile = tile_map(blockIdx.x, g.ntiles);
+	int ox = (tile & (g.ntx - 1)) * g.tx - g.M, oy = (tile >> g.lgntx) * g.ty - g.M;
+	tile_setup<T>(g, ox, oy, (const typename V2<T>::t *) NULL, (typename V2<T>::t *) NULL, sR);
+	__syncthreads();
+	for(int s = 0; s < g.ns; s++)
+	{
+		int b = tile * g.ns + s;
+		int n1 = bucket_n1(a.bk, b), ntot = n1 + a.bk.n2[b];
+		T vq = (T) a.sp.vq[s];
+		for(int j = threadIdx.x; j < ntot; j += CPIC_BLOCK)
+		{
+			int64_t slot = bucket_slot(a.bk, b, n1, j);
+			CellPos c;
+			T rx, ry;
+			locate(g, ox, oy, ((const T *) a.p.x)[slot], ((const T *) a.p.y)[slot], c, rx, ry);
+			deposit<T>(g, sR, a.rho, c, rx, ry, vq);
+		}
+	}
+	__syncthreads();
+	tile_flush<T>(g, ox, oy, sR, a.rho);
+}
+
+// interpolate_f2p_E (src/interpolate.c:350-404): gathered E stored per
+// particle (only with CPIC_F_PARTICLE_E).
+template <typename T>
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_gather(PushArgs a)
+{
+	typedef typename V2<T>::t T2;
+	extern __shared__ unsigned char smem[];
+	const Geo &g = a.g;
+	T2 *sE = (T2 *) smem;
+	const T2 *__restrict__ E = (const T2 *) a.E;
+	int tile = tile_map(blockIdx.x, g.ntiles);
+	int ox = (tile & (g.ntx - 1)) * g.tx - g.M, oy = (tile >> g.lgntx) * g.ty - g.M;
+	tile_setup<T>(g, ox, oy, E, sE, (double *) NULL);
+	__syncthreads();
+	for(int s = 0; s < g.ns; s++)
+	{
+		int b = tile * g.ns + s;
+		int n1 = bucket_n1(a.bk, b), ntot = n1 + a.bk.n2[b];
+		for(int j = threadIdx.x; j < ntot; j += CPIC_BLOCK)
+		{
+			int64_t slot = bucket_slot(a.bk, b, n1, j);
+			CellPos c;
+			T rx, ry, ex, ey;
+			locate(g, ox, oy, ((const T *) a.p.x)[slot], ((const T *) a.p.y)[slot], c, rx, ry);
+			gather<T>(g, sE, E, c, rx, ry, ex, ey);
+			a.p.Ex[slot] = (double) ex;
+			a.p.Ey[slot] = (double) ey;
+		}
+	}
+}
+
+// particles per thread and chunk in k_push
+#define PUSH_U 4
 
 // Places leavers in their destination bucket, or back in the reserved
 // slots of their source bucket when the destination is full.
@@ -863,6 +952,15 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 		else hipLaunchKernelGGL((kern<T, false, false>), __VA_ARGS__); \
 	} while(0)
 
+#define DISPATCH_U(kern, T, ...) do { \
+		bool ids_ = (d->desc.flags & CPIC_F_IDS) != 0; \
+		bool pe_ = (d->desc.flags & CPIC_F_PARTICLE_E) != 0; \
+		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, PUSH_U>), __VA_ARGS__); \
+		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, PUSH_U>), __VA_ARGS__); \
+		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, PUSH_U>), __VA_ARGS__); \
+		else hipLaunchKernelGGL((kern<T, false, false, PUSH_U>), __VA_ARGS__); \
+	} while(0)
+
 #define DISPATCH_P(kern, ...) do { \
 		if(d->prec == 64) DISPATCH(kern, double, __VA_ARGS__); \
 		else DISPATCH(kern, float, __VA_ARGS__); \
@@ -953,7 +1051,7 @@ push_lds(const cpic_dev *d)
 {
 	size_t n = (size_t) d->g.W * d->g.H;
 	return d->prec == 64 ? n * (sizeof(double2) + sizeof(double))
-		: n * (sizeof(float2) + sizeof(float));
+		: n * (sizeof(float2) + sizeof(double));
 }
 
 int
@@ -977,7 +1075,10 @@ cpic_parts_push(cpic_dev *d, int64_t iter)
 	size_t lds = push_lds(d);
 
 	slot = cpic_kt_begin(d, "push_gather_deposit");
-	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+	if(d->prec == 64)
+		DISPATCH_U(k_push, double, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+	else
+		DISPATCH_U(k_push, float, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
@@ -995,7 +1096,11 @@ cpic_parts_deposit(cpic_dev *d, double *rho_out)
 {
 	PushArgs a = push_args(d, 1, MODE_DEPOSIT, rho_out);
 	int slot = cpic_kt_begin(d, "deposit");
-	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), push_lds(d), d->stream, a);
+	size_t lds = (size_t) d->g.W * d->g.H * sizeof(double);
+	if(d->prec == 64)
+		hipLaunchKernelGGL(k_deposit<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+	else
+		hipLaunchKernelGGL(k_deposit<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1006,7 +1111,11 @@ cpic_parts_gather(cpic_dev *d)
 {
 	PushArgs a = push_args(d, 1, MODE_GATHER, d->rho[d->rho_cur]);
 	int slot = cpic_kt_begin(d, "gather");
-	DISPATCH_P(k_push, dim3(d->g.ntiles), dim3(CPIC_BLOCK), push_lds(d), d->stream, a);
+	size_t lds = (size_t) d->g.W * d->g.H * (d->prec == 64 ? sizeof(double2) : sizeof(float2));
+	if(d->prec == 64)
+		hipLaunchKernelGGL(k_gather<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+	else
+		hipLaunchKernelGGL(k_gather<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1232,10 +1341,10 @@ cpic_parts_init(cpic_dev *d)
 {
 	(void) d;
 	const void *k[] = {
-		(const void *) k_push<double, false, false>, (const void *) k_push<double, true, false>,
-		(const void *) k_push<double, false, true>, (const void *) k_push<double, true, true>,
-		(const void *) k_push<float, false, false>, (const void *) k_push<float, true, false>,
-		(const void *) k_push<float, false, true>, (const void *) k_push<float, true, true>,
+		(const void *) k_push<double, false, false, PUSH_U>, (const void *) k_push<double, true, false, PUSH_U>,
+		(const void *) k_push<double, false, true, PUSH_U>, (const void *) k_push<double, true, true, PUSH_U>,
+		(const void *) k_push<float, false, false, PUSH_U>, (const void *) k_push<float, true, false, PUSH_U>,
+		(const void *) k_push<float, false, true, PUSH_U>, (const void *) k_push<float, true, true, PUSH_U>,
 	};
 	for(unsigned i = 0; i < sizeof(k) / sizeof(k[0]); i++)
 		HIPCHK(hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize,
