@@ -785,3 +785,6 @@ host_generate(const char *path, int sp, int64_t n, double *x, double *y, double 
 	cfg_destroy(&c);
 	return ret;
 }
+
+int64_t sim_get_iter(const sim_t *s) { return s->iter; }
+int64_t sim_get_cycles(const sim_t *s) { return s->cycles; }

@@ -115,6 +115,8 @@ sim_t *sim_load(const char *path, int quiet);
 /* One particle's state by species and global index (needs CPIC_F_IDS,
  * which sim_init sets when simulation.track_particles is true). */
 int sim_particle(sim_t *sim, int species, int64_t index, double out[6]);
+int64_t sim_get_iter(const sim_t *sim);
+int64_t sim_get_cycles(const sim_t *sim);
 
 /* Reads a configuration file and generates species sp (n particles) into
  * the given arrays without touching a device; for tests. */

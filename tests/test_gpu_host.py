@@ -1,0 +1,120 @@
+"""The C host driver on the GPU: the reference's own test programs
+(test/harmonic.c, test/constant-speed.c) replayed through sim_load/sim_step,
+and the `cpic` command line on configuration files."""
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "cpic_amd", "lib", "libcpic_host.so")
+BIN = os.path.join(ROOT, "cpic_amd", "bin", "cpic")
+CONF = os.path.join(ROOT, "tests", "golden", "conf")
+
+pytestmark = pytest.mark.gpu
+
+
+def host():
+    h = C.CDLL(LIB)
+    h.sim_load.restype = C.c_void_p
+    h.sim_load.argtypes = [C.c_char_p, C.c_int]
+    h.sim_step.argtypes = [C.c_void_p]
+    h.sim_free.argtypes = [C.c_void_p]
+    h.sim_particle.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double)]
+    h.sim_get_iter.restype = C.c_int64
+    h.sim_get_iter.argtypes = [C.c_void_p]
+    h.sim_get_cycles.restype = C.c_int64
+    h.sim_get_cycles.argtypes = [C.c_void_p]
+    return h
+
+
+def tracked_copy(src, d, extra=""):
+    text = open(src).read().replace("simulation = {", "simulation = {\n\ttrack_particles = true\n" + extra, 1)
+    path = os.path.join(d, os.path.basename(src))
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+def particle(h, s, sp, idx):
+    out = (C.c_double * 6)()
+    assert h.sim_particle(s, sp, idx, out) == 0
+    return list(out)
+
+
+def test_harmonic_through_host_driver(golden_dir):
+    """test/harmonic.c: after every sim_step, r[X] and E[X] of electron 0
+    match harm.r0x / harm.E0x."""
+    r0x = np.loadtxt(os.path.join(golden_dir, "harm.r0x"))
+    e0x = np.loadtxt(os.path.join(golden_dir, "harm.E0x"))
+    h = host()
+    with tempfile.TemporaryDirectory() as d:
+        s = h.sim_load(tracked_copy(os.path.join(CONF, "harmonic.conf"), d).encode(), 1)
+    assert s
+    assert h.sim_get_iter(s) == 0 and h.sim_get_cycles(s) == 1200
+    rs, es = [], []
+    while h.sim_get_iter(s) < h.sim_get_cycles(s):
+        assert h.sim_step(s) == 0
+        p = particle(h, s, 0, 0)
+        rs.append(p[0])
+        es.append(p[4])
+    h.sim_free(s)
+    np.testing.assert_allclose(rs, r0x, rtol=0, atol=5.1e-7)
+    np.testing.assert_allclose(es, e0x, rtol=6e-7, atol=0)
+
+
+def test_constant_speed_through_host_driver():
+    """test/constant-speed.c: |u - u0| <= 1e-10 at every step (5,000 of the
+    50,000 here; the full run is in test_gpu_parity)."""
+    h = host()
+    with tempfile.TemporaryDirectory() as d:
+        s = h.sim_load(tracked_copy(os.path.join(CONF, "constant-speed.conf"), d).encode(), 1)
+    assert s
+    u0 = particle(h, s, 0, 0)[2:4]
+    for _ in range(5000):
+        assert h.sim_step(s) == 0
+        if h.sim_get_iter(s) % 250 == 0:
+            u = particle(h, s, 0, 0)[2:4]
+            assert abs(u[0] - u0[0]) < 1e-10 and abs(u[1] - u0[1]) < 1e-10
+    h.sim_free(s)
+
+
+def test_cpic_cli_stats_line():
+    """cpic prints the reference's stats line when sampling (src/sim.c:465)."""
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(ROOT, "cpic_amd", "conf", "two-stream-64.conf")
+        text = open(src).read().replace("stop_SEM = 0.0", "stop_SEM = 0.5").replace(
+            "cycles = 200", "cycles = 40")
+        path = os.path.join(d, "c.conf")
+        with open(path, "w") as f:
+            f.write(text)
+        r = subprocess.run([BIN, "-q", path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("stats ")]
+    assert lines, r.stdout
+    fields = dict(kv.split("=") for kv in lines[-1].split()[1:])
+    assert set(fields) == {"iter", "last", "mean", "std", "sem", "rsem", "mem", "solver"}
+    assert "Simulation runs now" in r.stdout
+
+
+def test_cpic_cli_umax_abort_is_clean():
+    """A velocity above umax fails with a clear error and a non-zero exit
+    (the reference aborts in check_velocity, src/mover.c:123-136)."""
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(ROOT, "cpic_amd", "conf", "two-stream-64.conf")
+        text = open(src).read().replace("drift_velocity = [0.1, 0.0]", "drift_velocity = [200.0, 0.0]")
+        path = os.path.join(d, "c.conf")
+        with open(path, "w") as f:
+            f.write(text)
+        r = subprocess.run([BIN, "-q", path], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "max velocity exceeded" in r.stderr
+
+
+def test_cpic_cli_bad_conf():
+    r = subprocess.run([BIN, os.path.join(CONF, "mpi.conf")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and 'pblock_nmax' in r.stderr
