@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=50, help="0 skips the FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps over which FP32 and FP64 field energies are compared (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 1 GiB device-to-device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -207,7 +209,14 @@ def main():
         log(f"fp{prec}: {steps} steps in {el:.3f} s, {el / steps * 1e3:.3f} ms/step, {st}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
             log(f"   {k:24s} {n:6d} launches {ms / max(n, 1):9.4f} ms avg")
-        res[prec] = dict(elapsed=el, steps=steps, kt=kt, stats=st)
+        hist = None
+        if args.accuracy_steps > 0:
+            done = args.warmup + steps
+            if args.accuracy_steps > done:
+                d.run(done, args.accuracy_steps - done)   # untimed, same seed in both precisions
+            d.sync()
+            hist = d.energy()["field"][:args.accuracy_steps]
+        res[prec] = dict(elapsed=el, steps=steps, kt=kt, stats=st, hist=hist)
         d.close()
         del d
 
@@ -251,6 +260,18 @@ def main():
         out["fp64"] = {"value": v64, "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
                        "push_achieved_GBs": (ntot * 64 + ng * 24) / a64 / 1e9 if a64 else None}
         out["fp32_over_fp64"] = value / v64
+        h32, h64 = res[32]["hist"], r64["hist"]
+        if h32 is not None and h64 is not None and len(h32) and len(h64):
+            n = min(len(h32), len(h64))
+            rel = np.abs(h32[:n] - h64[:n]) / np.abs(h64[:n])
+            above = np.nonzero(rel > 1e-5)[0]
+            out["fp32_field_energy_error"] = {
+                "steps": int(n), "reference": "FP64 GPU run of the same seed",
+                "max_rel": float(rel.max()),
+                "max_rel_first_100": float(rel[:100].max()),
+                "first_step_above_1e-5": int(above[0]) if len(above) else None,
+                "energy_growth": float(h64[n - 1] / h64[0]),
+            }
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(w)

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "lib", "libcpic_hip.so")
 
 MAX_SPECIES = 8
-F_IDS, F_PARTICLE_E, F_PROFILE, F_NO_DIAG = 0x1, 0x2, 0x4, 0x8
+F_IDS, F_PARTICLE_E, F_PROFILE, F_NO_DIAG, F_NO_GRAPH = 0x1, 0x2, 0x4, 0x8, 0x10
 RHO, PHI, EX, EY = 0, 1, 2, 3
 FIELDS = {"rho": RHO, "phi": PHI, "Ex": EX, "Ey": EY}
 ERRORS = {0: "CPIC_OK", -1: "CPIC_EINVAL", -2: "CPIC_ENOMEM", -3: "CPIC_EUMAX",
@@ -27,7 +27,10 @@ EXPORTS = (
     "cpic_dev_run", "cpic_dev_sync", "cpic_dev_solve_xy", "cpic_solver_rho_size",
     "cpic_dev_download_field", "cpic_dev_upload_field", "cpic_dev_energy", "cpic_dev_migration",
     "cpic_dev_cell_counts", "cpic_dev_rebuild", "cpic_dev_stats", "cpic_dev_kernel_times",
-    "cpic_dev_reset_kernel_times",
+    "cpic_dev_reset_kernel_times", "cpic_batch_create", "cpic_batch_size",
+    "cpic_batch_upload_particles", "cpic_batch_download_particles", "cpic_batch_count",
+    "cpic_batch_download_field", "cpic_batch_upload_field", "cpic_batch_energy",
+    "cpic_batch_cell_counts",
 )
 
 
@@ -100,6 +103,16 @@ def lib():
         L.cpic_dev_stats.argtypes = [vp, P(Stats)]
         L.cpic_dev_kernel_times.argtypes = [vp, i32, P(C.c_char_p), i64, d]
         L.cpic_dev_reset_kernel_times.argtypes = [vp]
+        L.cpic_batch_create.argtypes = [P(Desc), C.c_int32, P(vp)]
+        L.cpic_batch_size.argtypes = [vp, i32]
+        L.cpic_batch_upload_particles.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, i64,
+                                                  d, d, d, d]
+        L.cpic_batch_download_particles.argtypes = [vp, C.c_int32, C.c_int32, P(Particles)]
+        L.cpic_batch_count.argtypes = [vp, C.c_int32, C.c_int32, i64]
+        L.cpic_batch_download_field.argtypes = [vp, C.c_int32, C.c_int32, d]
+        L.cpic_batch_upload_field.argtypes = [vp, C.c_int32, C.c_int32, d]
+        L.cpic_batch_energy.argtypes = [vp, C.c_int32, C.c_int64, d, d, d, d, i64]
+        L.cpic_batch_cell_counts.argtypes = [vp, C.c_int32, C.c_int32, i64]
         _lib = L
     return _lib
 
@@ -124,31 +137,49 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int64)) if a is not None else None
 
 
+def make_desc(nx, ny, L, dt, q, m, e0=1.0, B=(0.0, 0.0, 0.0), nchunks=1, precision=64,
+              tile=(0, 0), flags=0, slack=0.0, device=0):
+    """A cpic_dev_desc_t from the reference's configuration values."""
+    desc = Desc()
+    desc.nx, desc.ny = int(nx), int(ny)
+    desc.L[0], desc.L[1] = float(L[0]), float(L[1])
+    desc.dt, desc.e0 = float(dt), float(e0)
+    for i in range(3):
+        desc.B[i] = float(B[i])
+    desc.nspecies = len(q)
+    desc.precision = int(precision)
+    for i, (qq, mm) in enumerate(zip(q, m)):
+        desc.q[i], desc.m[i] = float(qq), float(mm)
+    desc.nchunks = int(nchunks)
+    desc.tile[0], desc.tile[1] = int(tile[0]), int(tile[1])
+    desc.flags = int(flags)
+    desc.slack = float(slack)
+    desc.device = int(device)
+    return desc
+
+
 class Device:
-    """One simulation on one GPU (a cpic_dev_t)."""
+    """One simulation on one GPU (a cpic_dev_t holding one plasma).
+
+    Methods that address one plasma take `plasma` (0 for a single run);
+    stage calls act on every plasma of the handle."""
 
     def __init__(self, nx, ny, L, dt, q, m, e0=1.0, B=(0.0, 0.0, 0.0), nchunks=1,
                  precision=64, tile=(0, 0), flags=0, slack=0.0, device=0):
-        desc = Desc()
-        desc.nx, desc.ny = int(nx), int(ny)
-        desc.L[0], desc.L[1] = float(L[0]), float(L[1])
-        desc.dt, desc.e0 = float(dt), float(e0)
-        for i in range(3):
-            desc.B[i] = float(B[i])
-        desc.nspecies = len(q)
-        desc.precision = int(precision)
-        for i, (qq, mm) in enumerate(zip(q, m)):
-            desc.q[i], desc.m[i] = float(qq), float(mm)
-        desc.nchunks = int(nchunks)
-        desc.tile[0], desc.tile[1] = int(tile[0]), int(tile[1])
-        desc.flags = int(flags)
-        desc.slack = float(slack)
-        desc.device = int(device)
+        desc = make_desc(nx, ny, L, dt, q, m, e0, B, nchunks, precision, tile, flags, slack,
+                         device)
         h = C.c_void_p()
         check(lib().cpic_dev_create(C.byref(desc), C.byref(h)))
+        self._init([desc], h)
+
+    def _init(self, descs, h):
         self._h = h
-        self.nx, self.ny, self.nspecies, self.flags = int(nx), int(ny), len(q), int(flags)
-        self.desc = desc
+        self.descs = list(descs)
+        self.desc = descs[0]
+        self.P = len(descs)
+        self.nx, self.ny = int(descs[0].nx), int(descs[0].ny)
+        self.nspecies = int(descs[0].nspecies)
+        self.flags = int(descs[0].flags)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -158,19 +189,19 @@ class Device:
     def __del__(self):
         self.close()
 
-    def upload(self, species, ids, x, y, ux, uy):
+    def upload(self, species, ids, x, y, ux, uy, plasma=0):
         ids = np.ascontiguousarray(ids, dtype=np.int64)
         x, y, ux, uy = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, ux, uy))
-        check(lib().cpic_dev_upload_particles(self._h, species, len(x), _ip(ids), _dp(x), _dp(y),
-                                              _dp(ux), _dp(uy)))
+        check(lib().cpic_batch_upload_particles(self._h, plasma, species, len(x), _ip(ids), _dp(x),
+                                                _dp(y), _dp(ux), _dp(uy)))
 
-    def count(self, species):
+    def count(self, species, plasma=0):
         n = C.c_int64()
-        check(lib().cpic_dev_count(self._h, species, C.byref(n)))
+        check(lib().cpic_batch_count(self._h, plasma, species, C.byref(n)))
         return n.value
 
-    def particles(self, species, sort=True):
-        n = self.count(species)
+    def particles(self, species, sort=True, plasma=0):
+        n = self.count(species, plasma)
         out = {k: np.empty(n) for k in ("x", "y", "ux", "uy")}
         out["cell"] = np.empty(n, np.int64)
         if self.flags & F_IDS:
@@ -186,7 +217,7 @@ class Device:
         p.cell = _ip(out["cell"])
         if "id" in out:
             p.id = _ip(out["id"])
-        check(lib().cpic_dev_download_particles(self._h, species, C.byref(p)))
+        check(lib().cpic_batch_download_particles(self._h, plasma, species, C.byref(p)))
         assert p.n == n, (p.n, n)
         if sort and "id" in out:
             order = np.argsort(out["id"], kind="stable")
@@ -211,19 +242,24 @@ class Device:
     def solve_xy(self):
         check(lib().cpic_dev_solve_xy(self._h))
 
-    def field(self, which):
-        out = np.empty((self.ny, self.nx))
-        check(lib().cpic_dev_download_field(self._h, FIELDS[which], _dp(out)))
+    def shape(self, plasma=0):
+        d = self.descs[plasma]
+        return int(d.ny), int(d.nx)
+
+    def field(self, which, plasma=0):
+        out = np.empty(self.shape(plasma))
+        check(lib().cpic_batch_download_field(self._h, plasma, FIELDS[which], _dp(out)))
         return out
 
-    def set_field(self, which, arr):
-        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.ny, self.nx)
-        check(lib().cpic_dev_upload_field(self._h, FIELDS[which], _dp(arr)))
+    def set_field(self, which, arr, plasma=0):
+        arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape(plasma))
+        check(lib().cpic_batch_upload_field(self._h, plasma, FIELDS[which], _dp(arr)))
 
-    def energy(self, maxn=1 << 16):
+    def energy(self, maxn=1 << 16, plasma=0):
         n = C.c_int64()
         f, k, px, py = (np.empty(maxn) for _ in range(4))
-        check(lib().cpic_dev_energy(self._h, maxn, _dp(f), _dp(k), _dp(px), _dp(py), C.byref(n)))
+        check(lib().cpic_batch_energy(self._h, plasma, maxn, _dp(f), _dp(k), _dp(px), _dp(py),
+                                      C.byref(n)))
         m = min(n.value, maxn)
         return {"field": f[:m], "kinetic": k[:m], "px": px[:m], "py": py[:m]}
 
@@ -241,10 +277,11 @@ class Device:
                                        dst.ctypes.data_as(i32)))
         return {"id": ids, "species": sp, "src": src, "dst": dst}
 
-    def cell_counts(self, species):
-        out = np.empty(self.nx * self.ny, np.int64)
-        check(lib().cpic_dev_cell_counts(self._h, species, _ip(out)))
-        return out.reshape(self.ny, self.nx)
+    def cell_counts(self, species, plasma=0):
+        ny, nx = self.shape(plasma)
+        out = np.empty(nx * ny, np.int64)
+        check(lib().cpic_batch_cell_counts(self._h, plasma, species, _ip(out)))
+        return out.reshape(ny, nx)
 
     def rebuild(self):
         check(lib().cpic_dev_rebuild(self._h))
@@ -265,6 +302,18 @@ class Device:
 
     def reset_kernel_times(self):
         check(lib().cpic_dev_reset_kernel_times(self._h))
+
+
+class Batch(Device):
+    """P independent plasmas in one handle (cpic_batch_create): `descs` is a
+    list of Desc (make_desc) sharing precision, flags, device and slack."""
+
+    def __init__(self, descs):
+        descs = list(descs)
+        arr = (Desc * len(descs))(*descs)
+        h = C.c_void_p()
+        check(lib().cpic_batch_create(arr, len(descs), C.byref(h)))
+        self._init(descs, h)
 
 
 def solver_rho_size(nx, ny):

@@ -5,6 +5,15 @@
 // field_rho.  rho is double-buffered: plasma_r deposits the moved particles
 // into rho[1-cur] and field_rho makes it current, so every stage observes
 // the same rho as the reference's stage of the same name.
+//
+// A state object holds a batch of P independent plasmas; every stage runs
+// all of them with one launch per kernel.  cpic_dev_create makes a batch of
+// one, and the cpic_dev_* calls that address one plasma act on plasma 0.
+//
+// cpic_dev_run replays steps >= 1 two at a time from a captured HIP graph
+// (the leaver parity and the rho buffers alternate each step, so two steps
+// return every pointer to where it was).  The step counter lives on the
+// device, so a replay needs no new kernel arguments.
 #include "dev.h"
 
 #include <cmath>
@@ -38,7 +47,7 @@ int
 cpic_kt_begin(cpic_dev *d, const char *name)
 {
 	int k;
-	if(!(d->desc.flags & CPIC_F_PROFILE))
+	if(!(d->flags & CPIC_F_PROFILE))
 		return -1;
 	for(k = 0; k < d->nkt; k++)
 		if(strcmp(d->kt[k].name, name) == 0)
@@ -56,13 +65,15 @@ cpic_kt_begin(cpic_dev *d, const char *name)
 	{
 		int nc = d->evcap ? 2 * d->evcap : 256;
 		hipEvent_t *ev = (hipEvent_t *) realloc(d->ev, sizeof(hipEvent_t) * nc);
-		int *ek = (int *) realloc(d->ev_kernel, sizeof(int) * nc);
-		if(!ev || !ek)
+		if(!ev)
 			return -1;
+		d->ev = ev;
+		int *ek = (int *) realloc(d->ev_kernel, sizeof(int) * nc);
+		if(!ek)
+			return -1;
+		d->ev_kernel = ek;
 		for(int i = d->evcap; i < nc; i++)
 			hipEventCreate(&ev[i]);
-		d->ev = ev;
-		d->ev_kernel = ek;
 		d->evcap = nc;
 	}
 	int slot = d->nev;
@@ -101,6 +112,15 @@ kt_collect(cpic_dev *d)
 
 // -------------------------------------------------------------- setup ---
 
+// Makes the handle's device current (not while a step is being captured:
+// the stream is then recording and the device is already current).
+static void
+set_device(cpic_dev *d)
+{
+	if(!d->capturing)
+		hipSetDevice(d->device);
+}
+
 static int
 alloc_zero(void **p, size_t bytes)
 {
@@ -112,12 +132,22 @@ alloc_zero(void **p, size_t bytes)
 }
 
 static void
+graph_drop(cpic_dev *d)
+{
+	if(d->graph)
+		hipGraphExecDestroy(d->graph);
+	d->graph = NULL;
+	d->graph_ok = 0;
+}
+
+static void
 dev_free(cpic_dev *d)
 {
 	if(!d)
 		return;
 	if(d->stream)
 		hipStreamSynchronize(d->stream);
+	graph_drop(d);
 	hipFree(d->rho[0]); hipFree(d->rho[1]); hipFree(d->ghat); hipFree(d->phi);
 	hipFree(d->E); hipFree(d->costab); hipFree(d->tw);
 	cpic_parts_free(d, 0);
@@ -127,12 +157,21 @@ dev_free(cpic_dev *d)
 	hipFree(d->lv.x); hipFree(d->lv.y); hipFree(d->lv.ux); hipFree(d->lv.uy);
 	hipFree(d->lv.id); hipFree(d->lv.Ex); hipFree(d->lv.Ey); hipFree(d->lv.dst);
 	hipFree(d->lv.src); hipFree(d->lv.count);
-	hipFree(d->diag); hipFree(d->derr); hipFree(d->ctr);
-	for(int s = 0; s < CPIC_MAX_SPECIES; s++)
-	{
-		free(d->up_buf[s]);
-		free(d->up_id[s]);
-	}
+	hipFree(d->diag); hipFree(d->derr); hipFree(d->ctr); hipFree(d->st);
+	hipFree(d->geo); hipFree(d->pre);
+	if(d->up_buf)
+		for(int k = 0; k < d->P * CPIC_MAX_SPECIES; k++)
+		{
+			free(d->up_buf[k]);
+			free(d->up_id[k]);
+		}
+	free(d->up_buf);
+	free(d->up_id);
+	free(d->up_n);
+	free(d->count);
+	free(d->desc);
+	free(d->hgeo);
+	free(d->hpre);
 	for(int i = 0; i < d->evcap; i++)
 		hipEventDestroy(d->ev[i]);
 	free(d->ev);
@@ -147,6 +186,7 @@ leavers_alloc(cpic_dev *d, int64_t cap)
 {
 	Leavers *l = &d->lv;
 	size_t e = d->prec == 64 ? sizeof(double) : sizeof(float);
+	graph_drop(d);
 	hipFree(l->x); hipFree(l->y); hipFree(l->ux); hipFree(l->uy);
 	hipFree(l->id); hipFree(l->Ex); hipFree(l->Ey); hipFree(l->dst); hipFree(l->src);
 	l->x = l->y = l->ux = l->uy = NULL;
@@ -160,9 +200,9 @@ leavers_alloc(cpic_dev *d, int64_t cap)
 			hipMalloc(&l->dst, sizeof(int32_t) * cap) != hipSuccess ||
 			hipMalloc(&l->src, sizeof(int32_t) * cap) != hipSuccess)
 		return CPIC_ENOMEM;
-	if((d->desc.flags & CPIC_F_IDS) && hipMalloc(&l->id, sizeof(int64_t) * cap) != hipSuccess)
+	if((d->flags & CPIC_F_IDS) && hipMalloc(&l->id, sizeof(int64_t) * cap) != hipSuccess)
 		return CPIC_ENOMEM;
-	if((d->desc.flags & CPIC_F_PARTICLE_E) &&
+	if((d->flags & CPIC_F_PARTICLE_E) &&
 			(hipMalloc(&l->Ex, sizeof(double) * cap) != hipSuccess ||
 			 hipMalloc(&l->Ey, sizeof(double) * cap) != hipSuccess))
 		return CPIC_ENOMEM;
@@ -170,21 +210,16 @@ leavers_alloc(cpic_dev *d, int64_t cap)
 	return 0;
 }
 
-extern "C" int
-cpic_dev_create(const cpic_dev_desc_t *desc, cpic_dev_t **out)
+// sim_prepare's checks (src/sim.c:110-130) and this implementation's limits.
+static int
+desc_check(const cpic_dev_desc_t *desc)
 {
-	cpic_dev *d;
-	int ndev = 0, ret, s;
-	int64_t nx, ny, tx, ty;
-
-	if(!desc || !out)
-		return CPIC_EINVAL;
-	*out = NULL;
-	nx = desc->nx;
-	ny = desc->ny;
+	int64_t nx = desc->nx, ny = desc->ny, tx, ty;
 	if(!is_pow2(nx) || !is_pow2(ny) || nx < 2 || ny < 2 || nx > 8192 || ny > 8192)
 		return CPIC_EINVAL;
-	if(!(desc->L[0] > 0) || !(desc->L[1] > 0) || !(desc->dt > 0) || desc->e0 == 0.0)
+	if(!(desc->L[0] > 0) || !(desc->L[1] > 0) || !(desc->dt > 0) || desc->e0 == 0.0 ||
+			!std::isfinite(desc->L[0]) || !std::isfinite(desc->L[1]) ||
+			!std::isfinite(desc->dt) || !std::isfinite(desc->e0) || !std::isfinite(desc->B[2]))
 		return CPIC_EINVAL;
 	if(desc->nspecies < 1 || desc->nspecies > CPIC_MAX_SPECIES)
 		return CPIC_EINVAL;
@@ -196,8 +231,8 @@ cpic_dev_create(const cpic_dev_desc_t *desc, cpic_dev_t **out)
 		return CPIC_EINVAL;
 	if(desc->nchunks < 1 || nx % desc->nchunks)   // src/sim.c:124-130
 		return CPIC_EINVAL;
-	for(s = 0; s < desc->nspecies; s++)
-		if(!(desc->m[s] != 0.0) || !std::isfinite(desc->q[s]))
+	for(int s = 0; s < desc->nspecies; s++)
+		if(!(desc->m[s] != 0.0) || !std::isfinite(desc->m[s]) || !std::isfinite(desc->q[s]))
 			return CPIC_EINVAL;
 	tx = desc->tile[0] > 0 ? desc->tile[0] : 16;
 	ty = desc->tile[1] > 0 ? desc->tile[1] : 16;
@@ -205,66 +240,182 @@ cpic_dev_create(const cpic_dev_desc_t *desc, cpic_dev_t **out)
 	if(ty > ny) ty = ny;
 	if(!is_pow2(tx) || !is_pow2(ty) || tx > 64 || ty > 64)
 		return CPIC_EINVAL;
+	return 0;
+}
 
+// Table entry of one plasma (offsets are filled in by the caller).
+static void
+geo_fill(const cpic_dev_desc_t *desc, Geo *g)
+{
+	int64_t nx = desc->nx, ny = desc->ny;
+	int64_t tx = desc->tile[0] > 0 ? desc->tile[0] : 16;
+	int64_t ty = desc->tile[1] > 0 ? desc->tile[1] : 16;
+	if(tx > nx) tx = nx;
+	if(ty > ny) ty = ny;
+	memset(g, 0, sizeof(*g));
+	g->nx = (int) nx;
+	g->ny = (int) ny;
+	g->lgnx = ilog2(nx);
+	g->lgny = ilog2(ny);
+	g->tx = (int) tx;
+	g->ty = (int) ty;
+	g->lgtx = ilog2(tx);
+	g->lgty = ilog2(ty);
+	g->ntx = (int) (nx / tx);
+	g->nty = (int) (ny / ty);
+	g->lgntx = ilog2(g->ntx);
+	g->ntiles = g->ntx * g->nty;
+	g->M = CPIC_MARGIN;
+	g->W = g->tx + 2 * g->M + 1;
+	g->H = g->ty + 2 * g->M + 1;
+	g->ns = desc->nspecies;
+	g->dx = desc->L[0] / nx;      // src/sim.c:174
+	g->dy = desc->L[1] / ny;
+	g->idx = 1.0 / g->dx;         // src/interpolate.c:365-366
+	g->idy = 1.0 / g->dy;
+	g->L0 = desc->L[0];
+	g->L1 = desc->L[1];
+	g->dt = desc->dt;
+	g->e0 = desc->e0;
+	g->Bz = desc->B[2];
+	// umax = chunksize * dx / dt (src/sim.c:198-200)
+	g->umax_x = (double) (nx / desc->nchunks) * g->dx / desc->dt;
+	g->umax_y = (double) ny * g->dy / desc->dt;
+	for(int s = 0; s < desc->nspecies; s++)
+	{
+		g->q[s] = desc->q[s];
+		g->m[s] = desc->m[s];
+	}
+	cpic_fft_blocking(g);
+}
+
+static int
+batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
+{
+	cpic_dev *d;
+	int ndev = 0, ret, p, k;
+	const int MS = CPIC_MAX_SPECIES;
+
+	if(!descs || !out || P < 1 || P > 65536)
+		return CPIC_EINVAL;
+	*out = NULL;
+	for(p = 0; p < P; p++)
+	{
+		if((ret = desc_check(&descs[p])))
+			return ret;
+		// one precision, flag set, device and slack for the whole batch
+		if(descs[p].precision != descs[0].precision || descs[p].flags != descs[0].flags ||
+				descs[p].device != descs[0].device || descs[p].slack != descs[0].slack)
+			return CPIC_EINVAL;
+	}
 	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
 		return CPIC_ENODEV;
-	if(desc->device < 0 || desc->device >= ndev)
+	if(descs[0].device < 0 || descs[0].device >= ndev)
 		return CPIC_EINVAL;
-	if(hipSetDevice(desc->device) != hipSuccess)
+	if(hipSetDevice(descs[0].device) != hipSuccess)
 		return CPIC_EHIP;
 
 	d = (cpic_dev *) calloc(1, sizeof(*d));
 	if(!d)
 		return CPIC_ENOMEM;
-	d->desc = *desc;
-	d->prec = desc->precision;
-	d->ns = desc->nspecies;
-	d->dt = desc->dt;
-	d->device = desc->device;
+	d->P = P;
+	d->prec = descs[0].precision;
+	d->flags = descs[0].flags;
+	d->device = descs[0].device;
+	d->ring = P <= 4 ? 65536 : (262144 / P > 4096 ? 262144 / P : 4096);
+	d->desc = (cpic_dev_desc_t *) malloc(sizeof(cpic_dev_desc_t) * P);
+	d->hgeo = (Geo *) calloc((size_t) P, sizeof(Geo));
+	d->hpre = (int *) calloc((size_t) PRE_COUNT * (P + 1), sizeof(int));
+	d->count = (int64_t *) calloc((size_t) P * MS, sizeof(int64_t));
+	d->up_n = (int64_t *) calloc((size_t) P * MS, sizeof(int64_t));
+	d->up_buf = (double **) calloc((size_t) P * MS, sizeof(double *));
+	d->up_id = (int64_t **) calloc((size_t) P * MS, sizeof(int64_t *));
+	if(!d->desc || !d->hgeo || !d->hpre || !d->count || !d->up_n || !d->up_buf || !d->up_id)
+	{
+		dev_free(d);
+		return CPIC_ENOMEM;
+	}
+	memcpy(d->desc, descs, sizeof(cpic_dev_desc_t) * P);
 
-	Geo &g = d->g;
-	g.nx = (int) nx;
-	g.ny = (int) ny;
-	g.lgnx = ilog2(nx);
-	g.lgny = ilog2(ny);
-	g.tx = (int) tx;
-	g.ty = (int) ty;
-	g.lgtx = ilog2(tx);
-	g.lgty = ilog2(ty);
-	g.ntx = (int) (nx / tx);
-	g.nty = (int) (ny / ty);
-	g.lgntx = ilog2(g.ntx);
-	g.ntiles = g.ntx * g.nty;
-	g.M = CPIC_MARGIN;
-	g.W = g.tx + 2 * g.M + 1;
-	g.H = g.ty + 2 * g.M + 1;
-	g.ns = d->ns;
-	g.dx = desc->L[0] / nx;      // src/sim.c:174
-	g.dy = desc->L[1] / ny;
-	g.idx = 1.0 / g.dx;          // src/interpolate.c:365-366
-	g.idy = 1.0 / g.dy;
-	g.L0 = desc->L[0];
-	g.L1 = desc->L[1];
-	d->nb = g.ntiles * d->ns;
-	// umax = chunksize * dx / dt (src/sim.c:198-200)
-	d->umax[0] = (double) (nx / desc->nchunks) * g.dx / desc->dt;
-	d->umax[1] = (double) ny * g.dy / desc->dt;
+	// table entries, work-list prefixes and totals
+	int64_t nodes = 0, ghat = 0, cosn = 0, tiles = 0, buckets = 0;
+	int64_t rows = 0, cols = 0, nblk = 0;
+	size_t lds_rows = 0, lds_cols = 0;
+	int *pt = d->hpre + PRE_TILE * (P + 1), *pr = d->hpre + PRE_ROW * (P + 1);
+	int *pc = d->hpre + PRE_COL * (P + 1), *pn = d->hpre + PRE_NODEBLK * (P + 1);
+	int *pb = d->hpre + PRE_BUCKET * (P + 1);
+	for(p = 0; p < P; p++)
+	{
+		Geo *g = &d->hgeo[p];
+		geo_fill(&descs[p], g);
+		int64_t n = (int64_t) g->nx * g->ny, hx = g->nx / 2 + 1;
+		g->node0 = nodes;
+		g->ghat0 = ghat;
+		g->cos0 = cosn;
+		g->tile0 = (int) tiles;
+		g->bucket0 = (int) buckets;
+		g->diag0 = p * d->ring;
+		pt[p] = (int) tiles;
+		pr[p] = (int) rows;
+		pc[p] = (int) cols;
+		pn[p] = (int) nblk;
+		pb[p] = (int) buckets;
+		nodes += n;
+		ghat += hx * g->ny;
+		cosn += hx + g->ny;
+		tiles += g->ntiles;
+		buckets += (int64_t) g->ntiles * g->ns;
+		rows += g->ny / g->R;
+		cols += (hx + g->C - 1) / g->C;
+		nblk += (n + EF_NODES - 1) / EF_NODES;
+		size_t lr = sizeof(double2) * (size_t) g->R * (size_t) hx;
+		size_t lc = sizeof(double2) * (size_t) g->C * (size_t) g->ny;
+		if(lr > lds_rows) lds_rows = lr;
+		if(lc > lds_cols) lds_cols = lc;
+		if(tiles > (1 << 30) || buckets > (1 << 30) || rows > (1 << 30) || cols > (1 << 30) ||
+				nblk > (1 << 30))
+		{
+			dev_free(d);
+			return CPIC_EINVAL;
+		}
+	}
+	pt[P] = (int) tiles;
+	pr[P] = (int) rows;
+	pc[P] = (int) cols;
+	pn[P] = (int) nblk;
+	pb[P] = (int) buckets;
+	d->nodes = nodes;
+	d->ghat_n = ghat;
+	d->ntiles = (int) tiles;
+	d->nb = (int) buckets;
+	d->rowblocks = (int) rows;
+	d->colblocks = (int) cols;
+	d->nodeblocks = (int) nblk;
+	d->lds_rows = lds_rows;
+	d->lds_cols = lds_cols;
+	d->lds_push = cpic_push_lds(d);
+	for(p = 0; p < P; p++)
+		if(d->hgeo[p].ns > d->maxns)
+			d->maxns = d->hgeo[p].ns;
 
 	if(hipStreamCreate(&d->stream) != hipSuccess)
 	{
 		free(d);
 		return CPIC_EHIP;
 	}
-	size_t ng = (size_t) nx * ny;
 	size_t esz = d->prec == 64 ? sizeof(double2) : sizeof(float2);
+	size_t ng = (size_t) nodes;
 	if((ret = alloc_zero((void **) &d->rho[0], sizeof(double) * ng)) ||
 			(ret = alloc_zero((void **) &d->rho[1], sizeof(double) * ng)) ||
-			(ret = alloc_zero((void **) &d->ghat, sizeof(double2) * (size_t) (nx / 2 + 1) * ny)) ||
+			(ret = alloc_zero((void **) &d->ghat, sizeof(double2) * (size_t) ghat)) ||
 			(ret = alloc_zero((void **) &d->phi, sizeof(double) * ng)) ||
 			(ret = alloc_zero((void **) &d->E, esz * ng)) ||
-			(ret = alloc_zero((void **) &d->diag, sizeof(DiagSlot) * CPIC_DIAG_RING)) ||
+			(ret = alloc_zero((void **) &d->diag, sizeof(DiagSlot) * (size_t) P * d->ring)) ||
 			(ret = alloc_zero((void **) &d->derr, sizeof(int))) ||
 			(ret = alloc_zero((void **) &d->ctr, sizeof(Counters))) ||
+			(ret = alloc_zero((void **) &d->st, sizeof(StepState))) ||
+			(ret = alloc_zero((void **) &d->geo, sizeof(Geo) * P)) ||
+			(ret = alloc_zero((void **) &d->pre, sizeof(int) * PRE_COUNT * (P + 1))) ||
 			(ret = alloc_zero((void **) &d->bk.start, sizeof(int64_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.cap, sizeof(int32_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.n1, sizeof(int32_t) * d->nb)) ||
@@ -277,10 +428,40 @@ cpic_dev_create(const cpic_dev_desc_t *desc, cpic_dev_t **out)
 		dev_free(d);
 		return ret;
 	}
+	if(hipMemcpy(d->geo, d->hgeo, sizeof(Geo) * P, hipMemcpyHostToDevice) != hipSuccess ||
+			hipMemcpy(d->pre, d->hpre, sizeof(int) * PRE_COUNT * (P + 1),
+				hipMemcpyHostToDevice) != hipSuccess)
+	{
+		dev_free(d);
+		return CPIC_EHIP;
+	}
+	(void) k;
 	d->rho_cur = 0;
 	d->rho_next_ready = 1;   // both buffers start zeroed
 	d->last_iter = -1;
+	d->iter = INT64_MIN;     // device counter not set yet
 	*out = d;
+	return 0;
+}
+
+extern "C" int
+cpic_dev_create(const cpic_dev_desc_t *desc, cpic_dev_t **out)
+{
+	return batch_create(desc, 1, out);
+}
+
+extern "C" int
+cpic_batch_create(const cpic_dev_desc_t *descs, int32_t nplasmas, cpic_dev_t **out)
+{
+	return batch_create(descs, nplasmas, out);
+}
+
+extern "C" int
+cpic_batch_size(cpic_dev_t *d, int32_t *nplasmas)
+{
+	if(!d || !nplasmas)
+		return CPIC_EINVAL;
+	*nplasmas = d->P;
 	return 0;
 }
 
@@ -289,45 +470,68 @@ cpic_dev_destroy(cpic_dev_t *d)
 {
 	if(!d)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	dev_free(d);
 	return 0;
 }
 
 // ---------------------------------------------------------- particles ---
 
+static int
+plasma_ok(const cpic_dev *d, int p)
+{
+	return d && p >= 0 && p < d->P;
+}
+
+extern "C" int
+cpic_batch_upload_particles(cpic_dev_t *d, int32_t p, int32_t s, int64_t n, const int64_t *id,
+		const double *x, const double *y, const double *ux, const double *uy)
+{
+	if(!plasma_ok(d, p) || s < 0 || s >= d->hgeo[p].ns || n < 0 ||
+			(n > 0 && (!x || !y || !ux || !uy)))
+		return CPIC_EINVAL;
+	if((d->flags & CPIC_F_IDS) && n > 0 && !id)
+		return CPIC_EINVAL;
+	const Geo &g = d->hgeo[p];
+	for(int64_t i = 0; i < n; i++)
+		if(!(x[i] >= 0.0 && x[i] <= g.L0 && y[i] >= 0.0 && y[i] <= g.L1) ||
+				!std::isfinite(ux[i]) || !std::isfinite(uy[i]))
+			return CPIC_EINVAL;
+	int k = p * CPIC_MAX_SPECIES + s;
+	free(d->up_buf[k]);
+	free(d->up_id[k]);
+	d->up_buf[k] = (double *) malloc(sizeof(double) * 4 * (n ? n : 1));
+	d->up_id[k] = NULL;
+	if(!d->up_buf[k])
+		return CPIC_ENOMEM;
+	memcpy(d->up_buf[k], x, sizeof(double) * n);
+	memcpy(d->up_buf[k] + n, y, sizeof(double) * n);
+	memcpy(d->up_buf[k] + 2 * n, ux, sizeof(double) * n);
+	memcpy(d->up_buf[k] + 3 * n, uy, sizeof(double) * n);
+	if(d->flags & CPIC_F_IDS)
+	{
+		d->up_id[k] = (int64_t *) malloc(sizeof(int64_t) * (n ? n : 1));
+		if(!d->up_id[k])
+			return CPIC_ENOMEM;
+		memcpy(d->up_id[k], id, sizeof(int64_t) * n);
+	}
+	d->up_n[k] = n;
+	d->pending = 1;
+	return 0;
+}
+
 extern "C" int
 cpic_dev_upload_particles(cpic_dev_t *d, int32_t s, int64_t n, const int64_t *id,
 		const double *x, const double *y, const double *ux, const double *uy)
 {
-	if(!d || s < 0 || s >= d->ns || n < 0 || (n > 0 && (!x || !y || !ux || !uy)))
-		return CPIC_EINVAL;
-	if((d->desc.flags & CPIC_F_IDS) && n > 0 && !id)
-		return CPIC_EINVAL;
-	for(int64_t i = 0; i < n; i++)
-		if(!(x[i] >= 0.0 && x[i] <= d->g.L0 && y[i] >= 0.0 && y[i] <= d->g.L1) ||
-				!std::isfinite(ux[i]) || !std::isfinite(uy[i]))
-			return CPIC_EINVAL;
-	free(d->up_buf[s]);
-	free(d->up_id[s]);
-	d->up_buf[s] = (double *) malloc(sizeof(double) * 4 * (n ? n : 1));
-	d->up_id[s] = NULL;
-	if(!d->up_buf[s])
-		return CPIC_ENOMEM;
-	memcpy(d->up_buf[s], x, sizeof(double) * n);
-	memcpy(d->up_buf[s] + n, y, sizeof(double) * n);
-	memcpy(d->up_buf[s] + 2 * n, ux, sizeof(double) * n);
-	memcpy(d->up_buf[s] + 3 * n, uy, sizeof(double) * n);
-	if(d->desc.flags & CPIC_F_IDS)
-	{
-		d->up_id[s] = (int64_t *) malloc(sizeof(int64_t) * (n ? n : 1));
-		if(!d->up_id[s])
-			return CPIC_ENOMEM;
-		memcpy(d->up_id[s], id, sizeof(int64_t) * n);
-	}
-	d->up_n[s] = n;
-	d->pending = 1;
-	return 0;
+	return cpic_batch_upload_particles(d, 0, s, n, id, x, y, ux, uy);
+}
+
+static int64_t
+species_count(const cpic_dev *d, int p, int s)
+{
+	int k = p * CPIC_MAX_SPECIES + s;
+	return d->up_buf[k] ? d->up_n[k] : d->count[k];
 }
 
 static int
@@ -337,9 +541,10 @@ commit(cpic_dev *d)
 	int64_t tot = 0;
 	if(!d->pending)
 		return 0;
-	hipSetDevice(d->device);
-	for(int s = 0; s < d->ns; s++)
-		tot += d->up_buf[s] ? d->up_n[s] : d->count[s];
+	set_device(d);
+	for(int p = 0; p < d->P; p++)
+		for(int s = 0; s < d->hgeo[p].ns; s++)
+			tot += species_count(d, p, s);
 	int64_t want = tot / 32;
 	if(want < 16384)
 		want = 16384;
@@ -370,47 +575,66 @@ cpic_dev_sync(cpic_dev_t *d)
 {
 	if(!d)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
 	return derr_check(d);
 }
 
 extern "C" int
-cpic_dev_count(cpic_dev_t *d, int32_t s, int64_t *n)
+cpic_batch_count(cpic_dev_t *d, int32_t p, int32_t s, int64_t *n)
 {
-	if(!d || s < 0 || s >= d->ns || !n)
+	if(!plasma_ok(d, p) || s < 0 || s >= d->hgeo[p].ns || !n)
 		return CPIC_EINVAL;
-	*n = d->up_buf[s] ? d->up_n[s] : d->count[s];
+	*n = species_count(d, p, s);
 	return 0;
 }
 
 extern "C" int
-cpic_dev_download_particles(cpic_dev_t *d, int32_t s, cpic_particles_t *out)
+cpic_dev_count(cpic_dev_t *d, int32_t s, int64_t *n)
+{
+	return cpic_batch_count(d, 0, s, n);
+}
+
+extern "C" int
+cpic_batch_download_particles(cpic_dev_t *d, int32_t p, int32_t s, cpic_particles_t *out)
 {
 	int ret;
-	if(!d || !out || s < 0 || s >= d->ns)
+	if(!plasma_ok(d, p) || !out || s < 0 || s >= d->hgeo[p].ns)
 		return CPIC_EINVAL;
-	if(out->id && !(d->desc.flags & CPIC_F_IDS))
+	if(out->id && !(d->flags & CPIC_F_IDS))
 		return CPIC_EINVAL;
-	if((out->Ex || out->Ey) && !(d->desc.flags & CPIC_F_PARTICLE_E))
+	if((out->Ex || out->Ey) && !(d->flags & CPIC_F_PARTICLE_E))
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	hipSetDevice(d->device);
-	if((ret = cpic_parts_download(d, s, out)))
+	set_device(d);
+	if((ret = cpic_parts_download(d, p, s, out)))
 		return ret;
 	return derr_check(d);
 }
 
 extern "C" int
-cpic_dev_cell_counts(cpic_dev_t *d, int32_t s, int64_t *counts)
+cpic_dev_download_particles(cpic_dev_t *d, int32_t s, cpic_particles_t *out)
+{
+	return cpic_batch_download_particles(d, 0, s, out);
+}
+
+extern "C" int
+cpic_batch_cell_counts(cpic_dev_t *d, int32_t p, int32_t s, int64_t *counts)
 {
 	int ret;
-	if(!d || !counts || s < 0 || s >= d->ns)
+	if(!plasma_ok(d, p) || !counts || s < 0 || s >= d->hgeo[p].ns)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	return cpic_parts_cell_counts(d, s, counts);
+	set_device(d);
+	return cpic_parts_cell_counts(d, p, s, counts);
+}
+
+extern "C" int
+cpic_dev_cell_counts(cpic_dev_t *d, int32_t s, int64_t *counts)
+{
+	return cpic_batch_cell_counts(d, 0, s, counts);
 }
 
 extern "C" int
@@ -419,7 +643,7 @@ cpic_dev_rebuild(cpic_dev_t *d)
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	if(d->pending)
 		return commit(d);
 	if((ret = cpic_parts_rebuild(d)))
@@ -430,6 +654,15 @@ cpic_dev_rebuild(cpic_dev_t *d)
 
 // ------------------------------------------------------------- stages ---
 
+// Makes the device step counter equal to iter before work that reads it.
+static int
+want_iter(cpic_dev *d, int64_t iter)
+{
+	if(d->iter == iter)
+		return 0;
+	return cpic_set_iter(d, iter);
+}
+
 extern "C" int
 cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 {
@@ -438,9 +671,12 @@ cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	if((ret = cpic_solve(d, iter, 1)))
+	set_device(d);
+	if((ret = want_iter(d, iter)))
 		return ret;
-	return cpic_efield(d, iter);
+	if((ret = cpic_solve(d, 1)))
+		return ret;
+	return cpic_efield(d);
 }
 
 extern "C" int
@@ -451,7 +687,8 @@ cpic_dev_solve_xy(cpic_dev_t *d)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	return cpic_solve(d, -1, 0);
+	set_device(d);
+	return cpic_solve(d, 0);
 }
 
 extern "C" int
@@ -463,7 +700,7 @@ cpic_dev_plasma_E(cpic_dev_t *d, int64_t iter)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	if(d->desc.flags & CPIC_F_PARTICLE_E)
+	if(d->flags & CPIC_F_PARTICLE_E)
 		return cpic_parts_gather(d);
 	return 0;
 }
@@ -471,7 +708,7 @@ cpic_dev_plasma_E(cpic_dev_t *d, int64_t iter)
 static int
 zero_rho(cpic_dev *d, double *r)
 {
-	return cpic_zero(d, r, (int64_t) d->g.nx * d->g.ny);
+	return cpic_zero(d, r, d->nodes);
 }
 
 extern "C" int
@@ -486,10 +723,13 @@ cpic_dev_plasma_r(cpic_dev_t *d, int64_t iter)
 		return ret;
 	if(d->rho_pending)
 		return CPIC_ESTATE;    // plasma_r twice without field_rho
+	set_device(d);
 	if(!d->rho_next_ready)
 		if((ret = zero_rho(d, d->rho[1 - d->rho_cur])))
 			return ret;
-	if((ret = cpic_parts_push(d, iter)))
+	if((ret = want_iter(d, iter)))
+		return ret;
+	if((ret = cpic_parts_push(d, iter == 0)))
 		return ret;
 	d->rho_pending = 1;
 	d->rho_next_ready = 0;
@@ -514,6 +754,7 @@ cpic_dev_field_rho(cpic_dev_t *d, int64_t iter)
 		return 0;
 	}
 	// rho_reset + rho_update + ghost fold from the stored positions
+	set_device(d);
 	if((ret = zero_rho(d, d->rho[d->rho_cur])))
 		return ret;
 	return cpic_parts_deposit(d, d->rho[d->rho_cur]);
@@ -543,13 +784,83 @@ cpic_dev_step(cpic_dev_t *d, int64_t iter)
 	return cpic_dev_field_rho(d, iter);
 }
 
+// Steps iter and iter+1 from the captured graph, capturing it first if
+// needed.  Only called in the steady state: no pending upload, rho_next not
+// zeroed yet, no deposit pending, device counter == iter, iter >= 1.
+static int
+graph_steps(cpic_dev *d, int64_t iter)
+{
+	int ret;
+	if(d->graph_ok && (d->graph_parity != d->step_parity || d->graph_rho != d->rho_cur))
+		return 1;   // out of phase: caller runs one step eagerly
+	if(!d->graph_ok)
+	{
+		hipGraph_t g = NULL;
+		graph_drop(d);
+		d->graph_parity = d->step_parity;
+		d->graph_rho = d->rho_cur;
+		HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+		d->capturing = 1;
+		ret = cpic_dev_step(d, iter);
+		if(!ret)
+			ret = cpic_dev_step(d, iter + 1);
+		d->capturing = 0;
+		hipError_t e = hipStreamEndCapture(d->stream, &g);
+		if(ret || e != hipSuccess)
+		{
+			if(g)
+				hipGraphDestroy(g);
+			return ret ? ret : CPIC_EHIP;
+		}
+		e = hipGraphInstantiate(&d->graph, g, NULL, NULL, 0);
+		hipGraphDestroy(g);
+		if(e != hipSuccess)
+		{
+			d->graph = NULL;
+			fprintf(stderr, "cpic_hip: hipGraphInstantiate failed: %s\n",
+					hipGetErrorString(e));
+			return CPIC_EHIP;
+		}
+		d->graph_ok = 1;
+		// the capture advanced the host state by two steps; the launch
+		// below performs them
+	}
+	else
+	{
+		d->iter += 2;
+		d->last_iter = iter + 1;
+	}
+	HIPCHK(hipGraphLaunch(d->graph, d->stream));
+	return 0;
+}
+
 extern "C" int
 cpic_dev_run(cpic_dev_t *d, int64_t iter0, int64_t nsteps)
 {
 	int ret;
-	for(int64_t i = 0; i < nsteps; i++)
-		if((ret = cpic_dev_step(d, iter0 + i)))
+	if(!d || nsteps < 0)
+		return CPIC_EINVAL;
+	int graphs = !(d->flags & (CPIC_F_NO_GRAPH | CPIC_F_PROFILE));
+	for(int64_t i = 0; i < nsteps; )
+	{
+		int64_t it = iter0 + i;
+		if(graphs && it >= 1 && nsteps - i >= 2 && !d->pending && !d->rho_pending &&
+				!d->rho_next_ready && d->iter == it)
+		{
+			set_device(d);
+			ret = graph_steps(d, it);
+			if(ret < 0)
+				return ret;
+			if(ret == 0)
+			{
+				i += 2;
+				continue;
+			}
+		}
+		if((ret = cpic_dev_step(d, it)))
 			return ret;
+		i++;
+	}
 	return 0;
 }
 
@@ -568,21 +879,22 @@ cpic_solver_rho_size(int64_t nx, int64_t ny, int64_t *cnx, int64_t *cny)
 // ------------------------------------------------------------- fields ---
 
 extern "C" int
-cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
+cpic_batch_download_field(cpic_dev_t *d, int32_t p, int32_t which, double *out)
 {
-	int64_t n;
+	int64_t n, o;
 	int ret;
-	if(!d || !out)
+	if(!plasma_ok(d, p) || !out)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
 		return ret;
-	n = (int64_t) d->g.nx * d->g.ny;
-	hipSetDevice(d->device);
+	n = (int64_t) d->hgeo[p].nx * d->hgeo[p].ny;
+	o = d->hgeo[p].node0;
+	set_device(d);
 	if(which == CPIC_RHO)
-		HIPCHK(hipMemcpyAsync(out, d->rho[d->rho_cur], sizeof(double) * n,
+		HIPCHK(hipMemcpyAsync(out, d->rho[d->rho_cur] + o, sizeof(double) * n,
 				hipMemcpyDeviceToHost, d->stream));
 	else if(which == CPIC_PHI)
-		HIPCHK(hipMemcpyAsync(out, d->phi, sizeof(double) * n, hipMemcpyDeviceToHost,
+		HIPCHK(hipMemcpyAsync(out, d->phi + o, sizeof(double) * n, hipMemcpyDeviceToHost,
 				d->stream));
 	else if(which == CPIC_EX || which == CPIC_EY)
 	{
@@ -592,7 +904,8 @@ cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
 			double2 *h = (double2 *) malloc(sizeof(double2) * n);
 			if(!h)
 				return CPIC_ENOMEM;
-			hipMemcpyAsync(h, d->E, sizeof(double2) * n, hipMemcpyDeviceToHost, d->stream);
+			hipMemcpyAsync(h, (double2 *) d->E + o, sizeof(double2) * n, hipMemcpyDeviceToHost,
+					d->stream);
 			hipStreamSynchronize(d->stream);
 			for(int64_t i = 0; i < n; i++)
 				out[i] = c ? h[i].y : h[i].x;
@@ -603,7 +916,8 @@ cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
 			float2 *h = (float2 *) malloc(sizeof(float2) * n);
 			if(!h)
 				return CPIC_ENOMEM;
-			hipMemcpyAsync(h, d->E, sizeof(float2) * n, hipMemcpyDeviceToHost, d->stream);
+			hipMemcpyAsync(h, (float2 *) d->E + o, sizeof(float2) * n, hipMemcpyDeviceToHost,
+					d->stream);
 			hipStreamSynchronize(d->stream);
 			for(int64_t i = 0; i < n; i++)
 				out[i] = c ? h[i].y : h[i].x;
@@ -617,41 +931,50 @@ cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
 }
 
 extern "C" int
-cpic_dev_upload_field(cpic_dev_t *d, int32_t which, const double *in)
+cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
 {
-	int64_t n;
-	if(!d || !in)
+	return cpic_batch_download_field(d, 0, which, out);
+}
+
+extern "C" int
+cpic_batch_upload_field(cpic_dev_t *d, int32_t p, int32_t which, const double *in)
+{
+	int64_t n, o;
+	if(!plasma_ok(d, p) || !in)
 		return CPIC_EINVAL;
-	n = (int64_t) d->g.nx * d->g.ny;
-	hipSetDevice(d->device);
+	n = (int64_t) d->hgeo[p].nx * d->hgeo[p].ny;
+	o = d->hgeo[p].node0;
+	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
 	if(which == CPIC_RHO)
-		HIPCHK(hipMemcpy(d->rho[d->rho_cur], in, sizeof(double) * n, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d->rho[d->rho_cur] + o, in, sizeof(double) * n, hipMemcpyHostToDevice));
 	else if(which == CPIC_PHI)
-		HIPCHK(hipMemcpy(d->phi, in, sizeof(double) * n, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d->phi + o, in, sizeof(double) * n, hipMemcpyHostToDevice));
 	else if(which == CPIC_EX || which == CPIC_EY)
 	{
 		int c = which == CPIC_EX ? 0 : 1;
 		if(d->prec == 64)
 		{
 			double2 *h = (double2 *) malloc(sizeof(double2) * n);
+			double2 *E = (double2 *) d->E + o;
 			if(!h)
 				return CPIC_ENOMEM;
-			hipMemcpy(h, d->E, sizeof(double2) * n, hipMemcpyDeviceToHost);
+			hipMemcpy(h, E, sizeof(double2) * n, hipMemcpyDeviceToHost);
 			for(int64_t i = 0; i < n; i++)
 				if(c) h[i].y = in[i]; else h[i].x = in[i];
-			hipMemcpy(d->E, h, sizeof(double2) * n, hipMemcpyHostToDevice);
+			hipMemcpy(E, h, sizeof(double2) * n, hipMemcpyHostToDevice);
 			free(h);
 		}
 		else
 		{
 			float2 *h = (float2 *) malloc(sizeof(float2) * n);
+			float2 *E = (float2 *) d->E + o;
 			if(!h)
 				return CPIC_ENOMEM;
-			hipMemcpy(h, d->E, sizeof(float2) * n, hipMemcpyDeviceToHost);
+			hipMemcpy(h, E, sizeof(float2) * n, hipMemcpyDeviceToHost);
 			for(int64_t i = 0; i < n; i++)
 				if(c) h[i].y = (float) in[i]; else h[i].x = (float) in[i];
-			hipMemcpy(d->E, h, sizeof(float2) * n, hipMemcpyHostToDevice);
+			hipMemcpy(E, h, sizeof(float2) * n, hipMemcpyHostToDevice);
 			free(h);
 		}
 	}
@@ -660,21 +983,27 @@ cpic_dev_upload_field(cpic_dev_t *d, int32_t which, const double *in)
 	return 0;
 }
 
+extern "C" int
+cpic_dev_upload_field(cpic_dev_t *d, int32_t which, const double *in)
+{
+	return cpic_batch_upload_field(d, 0, which, in);
+}
+
 // -------------------------------------------------------- diagnostics ---
 
 extern "C" int
-cpic_dev_energy(cpic_dev_t *d, int64_t max, double *field, double *kinetic,
+cpic_batch_energy(cpic_dev_t *d, int32_t p, int64_t max, double *field, double *kinetic,
 		double *px, double *py, int64_t *n)
 {
 	int64_t steps, k;
 	DiagSlot *h;
-	if(!d || max < 0)
+	if(!plasma_ok(d, p) || max < 0)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
 	steps = d->last_iter + 1;
-	if(steps > CPIC_DIAG_RING)
-		steps = CPIC_DIAG_RING;
+	if(steps > d->ring)
+		steps = d->ring;
 	if(n)
 		*n = steps;
 	k = steps < max ? steps : max;
@@ -683,7 +1012,12 @@ cpic_dev_energy(cpic_dev_t *d, int64_t max, double *field, double *kinetic,
 	h = (DiagSlot *) malloc(sizeof(DiagSlot) * k);
 	if(!h)
 		return CPIC_ENOMEM;
-	HIPCHK(hipMemcpy(h, d->diag, sizeof(DiagSlot) * k, hipMemcpyDeviceToHost));
+	if(hipMemcpy(h, d->diag + d->hgeo[p].diag0, sizeof(DiagSlot) * k, hipMemcpyDeviceToHost)
+			!= hipSuccess)
+	{
+		free(h);
+		return CPIC_EHIP;
+	}
 	for(int64_t i = 0; i < k; i++)
 	{
 		if(field) field[i] = h[i].field;
@@ -696,6 +1030,30 @@ cpic_dev_energy(cpic_dev_t *d, int64_t max, double *field, double *kinetic,
 }
 
 extern "C" int
+cpic_dev_energy(cpic_dev_t *d, int64_t max, double *field, double *kinetic,
+		double *px, double *py, int64_t *n)
+{
+	return cpic_batch_energy(d, 0, max, field, kinetic, px, py, n);
+}
+
+// plasma owning global bucket b (host copy of the prefix array)
+static int
+bucket_plasma(const cpic_dev *d, int b)
+{
+	const int *pb = d->hpre + PRE_BUCKET * (d->P + 1);
+	int lo = 0, hi = d->P - 1;
+	while(lo < hi)
+	{
+		int mid = (lo + hi + 1) >> 1;
+		if(pb[mid] <= b)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
+extern "C" int
 cpic_dev_migration(cpic_dev_t *d, int64_t *n, int64_t *id, int32_t *species,
 		int32_t *src_tile, int32_t *dst_tile)
 {
@@ -703,9 +1061,9 @@ cpic_dev_migration(cpic_dev_t *d, int64_t *n, int64_t *id, int32_t *species,
 	int64_t k;
 	if(!d || !n)
 		return CPIC_EINVAL;
-	if(id && !(d->desc.flags & CPIC_F_IDS))
+	if(id && !(d->flags & CPIC_F_IDS))
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
 	HIPCHK(hipMemcpy(&cnt, d->lv.count + (1 - d->step_parity), sizeof(int32_t),
 			hipMemcpyDeviceToHost));
@@ -731,9 +1089,11 @@ cpic_dev_migration(cpic_dev_t *d, int64_t *n, int64_t *id, int32_t *species,
 		hipMemcpy(id, d->lv.id, sizeof(int64_t) * k, hipMemcpyDeviceToHost);
 	for(int64_t i = 0; i < k; i++)
 	{
-		if(species) species[i] = src[i] % d->ns;
-		if(src_tile) src_tile[i] = src[i] / d->ns;
-		if(dst_tile) dst_tile[i] = dst[i] / d->ns;
+		const Geo &gs = d->hgeo[bucket_plasma(d, src[i])];
+		const Geo &gd = d->hgeo[bucket_plasma(d, dst[i])];
+		if(species) species[i] = (src[i] - gs.bucket0) % gs.ns;
+		if(src_tile) src_tile[i] = gs.tile0 + (src[i] - gs.bucket0) / gs.ns;
+		if(dst_tile) dst_tile[i] = gd.tile0 + (dst[i] - gd.bucket0) / gd.ns;
 	}
 	free(src);
 	free(dst);
@@ -746,20 +1106,21 @@ cpic_dev_stats(cpic_dev_t *d, cpic_dev_stats_t *st)
 	Counters c;
 	if(!d || !st)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
 	HIPCHK(hipMemcpy(&c, d->ctr, sizeof(c), hipMemcpyDeviceToHost));
 	memset(st, 0, sizeof(*st));
-	for(int s = 0; s < d->ns; s++)
-		st->nparticles += d->up_buf[s] ? d->up_n[s] : d->count[s];
+	for(int p = 0; p < d->P; p++)
+		for(int s = 0; s < d->hgeo[p].ns; s++)
+			st->nparticles += species_count(d, p, s);
 	st->capacity = d->pcap[d->pcur];
 	st->nbuckets = d->nb;
 	st->leavers = (int64_t) c.leavers;
 	st->fallbacks = (int64_t) c.fallbacks;
 	st->misplaced = (int64_t) c.misplaced;
 	st->rebuilds = d->nrebuilds;
-	st->tile[0] = d->g.tx;
-	st->tile[1] = d->g.ty;
+	st->tile[0] = d->hgeo[0].tx;
+	st->tile[1] = d->hgeo[0].ty;
 	st->precision = d->prec;
 	return 0;
 }
@@ -770,7 +1131,7 @@ cpic_dev_kernel_times(cpic_dev_t *d, int32_t *n, const char **names, int64_t *la
 {
 	if(!d || !n)
 		return CPIC_EINVAL;
-	hipSetDevice(d->device);
+	set_device(d);
 	kt_collect(d);
 	int k = d->nkt < *n ? d->nkt : *n;
 	for(int i = 0; i < k; i++)

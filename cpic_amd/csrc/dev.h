@@ -1,17 +1,25 @@
 // Internal state of the gfx950 PIC step.  Not part of the C ABI.
 //
+// One state object holds P >= 1 independent plasmas (a single simulation is
+// a batch of one).  Every kernel runs once per stage for the whole batch over
+// a work list of (plasma, item): tiles for the particle kernels, row blocks
+// and column blocks for the FFT passes, node blocks for the E stencil.  Block
+// b finds its plasma by binary search in a prefix array of item counts and
+// reads the plasma's geometry from a device table.
+//
 // Layout in HBM
-//   fields   rho[2] (double, nx*ny, double-buffered: the solve reads one while
-//            the push deposits into the other), ghat ((nx/2+1)*ny complex,
-//            stored kx-major so the column pass reads contiguous columns),
-//            phi (double nx*ny), E (double2 or float2 nx*ny, (Ex,Ey) pairs).
-//   particles tile-major buckets (bucket b = tile*nspecies + species), SoA
-//            arrays x, y, ux, uy (+ optional id, Ex, Ey).  A bucket owns
-//            slots [start, start+cap): segment 1 holds n1 particles at
+//   fields   rho[2] (double, the plasmas' nx*ny grids back to back,
+//            double-buffered: the solve reads one while the push deposits into
+//            the other), ghat ((nx/2+1)*ny complex per plasma, stored kx-major
+//            so the column pass reads contiguous columns), phi (double),
+//            E (double2 or float2 (Ex,Ey) pairs).
+//   particles buckets, one per (plasma, tile, species), tile-major inside a
+//            plasma; SoA arrays x, y, ux, uy (+ optional id, Ex, Ey).  A bucket
+//            owns slots [start, start+cap): segment 1 holds n1 particles at
 //            [0, n1), segment 2 holds n2 fallback particles at
-//            [cap-res, cap-res+n2), where res is the number of particles
-//            that left the bucket in the last push (their slots are kept free
-//            so a leaver whose destination is full can always return).
+//            [cap-res, cap-res+n2), where res is the number of particles that
+//            left the bucket in the last push (their slots are kept free so a
+//            leaver whose destination is full can always return).
 //   FP64 mode stores absolute positions (the reference's r); FP32 mode
 //   stores positions in cell units relative to the bucket tile's origin.
 #pragma once
@@ -24,15 +32,15 @@
 
 #define CPIC_BLOCK 256
 #define CPIC_MARGIN 1            // cells of drift covered by the LDS tile regions
-#define CPIC_DIAG_RING 65536     // per-step diagnostic slots
-#define CPIC_MAX_KERNELS 16
+#define CPIC_MAX_KERNELS 24
 
 // device error word bits
 #define CPIC_DERR_UMAX 0x1
 #define CPIC_DERR_NAN 0x2
 
+// Geometry and constants of one plasma (device table entry).
 struct Geo {
-	int nx, ny;          // grid points
+	int nx, ny;          // grid points (powers of two)
 	int lgnx, lgny;
 	int tx, ty;          // tile edge in cells
 	int lgtx, lgty;
@@ -42,18 +50,24 @@ struct Geo {
 	int W, H;            // LDS region: (tx + 2M + 1) x (ty + 2M + 1) nodes
 	int M;
 	int ns;              // species
+	int R, C;            // FFT rows per row block, columns per column block
 	double dx, dy;       // cell size
 	double idx, idy;     // 1/dx, 1/dy
 	double L0, L1;       // simulation.space_length
+	double dt, e0, Bz;
+	double umax_x, umax_y;
+	double q[CPIC_MAX_SPECIES], m[CPIC_MAX_SPECIES];
+	int64_t node0;       // first node in rho/phi/E
+	int64_t ghat0;       // first element of ghat
+	int64_t cos0;        // first element of costab
+	int tile0;           // first global tile
+	int bucket0;         // first global bucket
+	int diag0;           // first slot of this plasma's diagnostic ring
+	int pad;
 };
 
-struct Species {
-	double h[CPIC_MAX_SPECIES];    // 0.5*dt'*q/m
-	double tz[CPIC_MAX_SPECIES];   // B_z * h
-	double sz[CPIC_MAX_SPECIES];   // 2 tz / (1 + tz^2)
-	double vq[CPIC_MAX_SPECIES];   // -q/e0 (deposit constant)
-	double m[CPIC_MAX_SPECIES];
-};
+// Work-list prefix arrays (P+1 entries each), concatenated.
+enum { PRE_TILE = 0, PRE_ROW, PRE_COL, PRE_NODEBLK, PRE_BUCKET, PRE_COUNT };
 
 struct Buckets {
 	int64_t *start;
@@ -91,6 +105,14 @@ struct DiagSlot {
 	double px, py;
 };
 
+// Device-side step counter (the reference's sim->iter): set by the host at
+// pre_step, advanced by the last kernel of every step, read by the kernels
+// that record diagnostics, so captured steps replay without new arguments.
+struct StepState {
+	long long iter;
+	long long pad[7];
+};
+
 struct KernelTimer {
 	const char *name;
 	int64_t launches;
@@ -98,14 +120,25 @@ struct KernelTimer {
 };
 
 struct cpic_dev {
-	cpic_dev_desc_t desc;
-	Geo g;
+	int P;                   // plasmas
+	cpic_dev_desc_t *desc;   // [P]
+	Geo *hgeo;               // [P] host copy
+	Geo *geo;                // [P] device
+	int *hpre, *pre;         // [PRE_COUNT][P+1] host / device
 	int prec;                // 32 or 64
-	int ns;
-	double dt;
-	double umax[2];
+	int flags;
 	int device;
+	int maxns;
 	hipStream_t stream;
+	int ring;                // diagnostic slots per plasma
+
+	// totals
+	int64_t nodes;           // sum nx*ny
+	int64_t ghat_n;          // sum (nx/2+1)*ny
+	int ntiles;              // sum of tiles
+	int nb;                  // sum of buckets
+	int rowblocks, colblocks, nodeblocks;
+	size_t lds_push, lds_rows, lds_cols;
 
 	// fields
 	double *rho[2];
@@ -115,8 +148,8 @@ struct cpic_dev {
 	double2 *ghat;
 	double *phi;
 	void *E;                 // double2 (FP64) or float2 (FP32)
-	double *costab;          // cos(2 pi k/nx) k<=nx/2, then cos(2 pi k/ny) k<ny
-	double2 *tw;             // exp(-2 pi i k/N), k <= N/2, N = max(nx, ny)
+	double *costab;          // per plasma: cos(2 pi k/nx) k<=nx/2, then cos(2 pi k/ny) k<ny
+	double2 *tw;             // exp(-2 pi i k/N), k <= N/2, N = max over plasmas
 	int lgN;
 
 	// particles
@@ -124,24 +157,32 @@ struct cpic_dev {
 	int pcur;
 	int64_t pcap[2];
 	Buckets bk;
-	int nb;
 	Leavers lv;
 	int step_parity;
-	int64_t count[CPIC_MAX_SPECIES];
+	int64_t *count;          // [P * CPIC_MAX_SPECIES]
 
 	// pending uploads (host copies, applied by the next rebuild)
 	int pending;
-	int64_t up_n[CPIC_MAX_SPECIES];
-	double *up_buf[CPIC_MAX_SPECIES];   // x,y,ux,uy interleaved as 4 arrays
-	int64_t *up_id[CPIC_MAX_SPECIES];
+	int64_t *up_n;           // [P * CPIC_MAX_SPECIES]
+	double **up_buf;         // x,y,ux,uy as 4 arrays
+	int64_t **up_id;
 
 	// diagnostics and errors
-	DiagSlot *diag;          // device ring
+	DiagSlot *diag;          // device ring, P * ring slots
 	int *derr;               // device error word
 	Counters *ctr;           // device counters
-	int64_t last_iter;
+	StepState *st;           // device step counter
+	int64_t iter;            // host mirror of the step counter (next step)
+	int64_t last_iter;       // last iteration pushed
 	int64_t nrebuilds;
-	int pushed_since_rebuild;
+
+	// captured steps: a graph of two consecutive steps (leaver parity and
+	// rho buffer alternate each step), valid for steps >= 1
+	hipGraphExec_t graph;
+	int graph_parity;        // step_parity at capture
+	int graph_rho;           // rho_cur at capture
+	int graph_ok;
+	int capturing;           // inside hipStreamBeginCapture/EndCapture
 
 	// profiling
 	KernelTimer kt[CPIC_MAX_KERNELS];
@@ -153,18 +194,21 @@ struct cpic_dev {
 
 // helpers implemented in the .hip files
 int cpic_fft_init(cpic_dev *d);
-int cpic_solve(cpic_dev *d, int64_t iter, int want_energy);
-int cpic_efield(cpic_dev *d, int64_t iter);
+void cpic_fft_blocking(Geo *g);
+size_t cpic_push_lds(const cpic_dev *d);
+int cpic_set_iter(cpic_dev *d, int64_t iter);
+int cpic_solve(cpic_dev *d, int want_energy);
+int cpic_efield(cpic_dev *d);
+int cpic_parts_init(cpic_dev *d);
 int cpic_parts_rebuild(cpic_dev *d);
-int cpic_parts_push(cpic_dev *d, int64_t iter);
+int cpic_parts_push(cpic_dev *d, int first_iter);
 int cpic_parts_deposit(cpic_dev *d, double *rho_out);
 int cpic_parts_gather(cpic_dev *d);
-int cpic_parts_download(cpic_dev *d, int species, cpic_particles_t *out);
-int cpic_parts_cell_counts(cpic_dev *d, int species, int64_t *counts);
+int cpic_parts_download(cpic_dev *d, int plasma, int species, cpic_particles_t *out);
+int cpic_parts_cell_counts(cpic_dev *d, int plasma, int species, int64_t *counts);
 int cpic_parts_alloc(cpic_dev *d, int which, int64_t cap);
 void cpic_parts_free(cpic_dev *d, int which);
 int cpic_zero(cpic_dev *d, double *p, int64_t n);
-int cpic_parts_init(cpic_dev *d);
 
 // profiling wrappers (api.hip)
 int cpic_kt_begin(cpic_dev *d, const char *name);
@@ -184,6 +228,10 @@ void cpic_kt_end(cpic_dev *d, int slot);
 // Dynamic LDS limit requested for kernels with large tiles or rows: the
 // 160 KB of a gfx950 workgroup minus room for static LDS.
 #define CPIC_LDS_LIMIT (160 * 1024 - 8 * 1024)
+// LDS budget used to choose FFT rows/columns per block
+#define FFT_LDS_BYTES (64 * 1024)
+// grid nodes per block of the E stencil
+#define EF_NODES 2048
 
 static inline int ilog2(int64_t n)
 {
@@ -196,4 +244,19 @@ static inline int ilog2(int64_t n)
 static inline int is_pow2(int64_t n)
 {
 	return n > 0 && (n & (n - 1)) == 0;
+}
+
+// plasma owning work item x: pre[p] <= x < pre[p+1]
+static __device__ __forceinline__ int find_plasma(const int *__restrict__ pre, int P, int x)
+{
+	int lo = 0, hi = P - 1;
+	while(lo < hi)
+	{
+		int mid = (lo + hi + 1) >> 1;
+		if(pre[mid] <= x)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
 }

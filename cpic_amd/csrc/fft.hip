@@ -21,15 +21,16 @@
 // zeroes the rho buffer the next push deposits into.
 //
 // FFT stages are radix-2 in LDS with twiddles from one table of
-// exp(-2 pi i k/N), N = max(nx, ny), built on the host with an octant
-// reduction so the quarter points are exact.
+// exp(-2 pi i k/N), N = the largest grid edge of the batch, built on the host
+// with an octant reduction so the quarter points are exact.  Each pass runs
+// once for all plasmas of a batch: block b takes the b-th item of the
+// pass's work list (row block, column block or node block) and finds its
+// plasma by binary search.
 #include "dev.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
-
-#define FFT_LDS_BYTES (64 * 1024)
 
 static __device__ __forceinline__ int brev(int j, int lg)
 {
@@ -100,15 +101,19 @@ static __device__ void fft_dif(double2 *buf, int stride, int lgn, int nbat,
 
 // Pass 1: R rows per block; row y real of length nx = 2m.
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_rows_r2c(const double *__restrict__ rho, double2 *__restrict__ ghat,
-		int nx, int ny, int R, const double2 *__restrict__ tw, int lgN)
+void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
+		const double *__restrict__ rho_all, double2 *__restrict__ ghat_all,
+		const double2 *__restrict__ tw, int lgN)
 {
 	extern __shared__ double2 sh[];
-	int m = nx >> 1, lgm = 0;
-	while((1 << lgm) < m)
-		lgm++;
+	int p = find_plasma(pre, P, blockIdx.x);
+	const Geo &g = geo[p];
+	int nx = g.nx, ny = g.ny, R = g.R;
+	const double *__restrict__ rho = rho_all + g.node0;
+	double2 *__restrict__ ghat = ghat_all + g.ghat0;
+	int m = nx >> 1, lgm = g.lgnx - 1;
 	int stride = m + 1;
-	int y0 = blockIdx.x * R;
+	int y0 = (blockIdx.x - pre[p]) * R;
 
 	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
 	{
@@ -165,19 +170,26 @@ void k_rows_r2c(const double *__restrict__ rho, double2 *__restrict__ ghat,
 
 // Pass 2: C columns (kx) per block, each ny long and contiguous.
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_cols(double2 *__restrict__ ghat, int nx, int ny, int C,
-		const double *__restrict__ costab, const double2 *__restrict__ tw,
-		int lgN, double invN, double *__restrict__ energy, double escale)
+void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
+		double2 *__restrict__ ghat_all, const double *__restrict__ costab_all,
+		const double2 *__restrict__ tw, int lgN, DiagSlot *__restrict__ diag,
+		int ring, const StepState *__restrict__ st)
 {
 	extern __shared__ double2 sh[];
 	__shared__ double red[CPIC_BLOCK];
+	int p = find_plasma(pre, P, blockIdx.x);
+	const Geo &g = geo[p];
+	int nx = g.nx, ny = g.ny, C = g.C;
+	double2 *__restrict__ ghat = ghat_all + g.ghat0;
 	int m = nx >> 1;
-	int lgny = 0;
-	while((1 << lgny) < ny)
-		lgny++;
-	int k0 = blockIdx.x * C;
+	int lgny = g.lgny;
+	int k0 = (blockIdx.x - pre[p]) * C;
 	int ncol = min(C, m + 1 - k0);
-	const double *cosx = costab, *cosy = costab + (m + 1);
+	const double *cosx = costab_all + g.cos0, *cosy = cosx + (m + 1);
+	double invN = 1.0 / ((double) nx * (double) ny);
+	long long iter = st->iter;
+	double *energy = (diag && iter >= 0) ? &diag[g.diag0 + iter % ring].field : (double *) NULL;
+	double escale = -0.5 * g.e0 * invN;
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
 	{
@@ -229,15 +241,19 @@ void k_cols(double2 *__restrict__ ghat, int nx, int ny, int C,
 // Pass 3: R rows per block, c2r along X (imaginary parts of X[0] and
 // X[nx/2] ignored, as for a Hermitian input).
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_rows_c2r(const double2 *__restrict__ ghat, double *__restrict__ phi,
-		int nx, int ny, int R, const double2 *__restrict__ tw, int lgN)
+void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
+		const double2 *__restrict__ ghat_all, double *__restrict__ phi_all,
+		const double2 *__restrict__ tw, int lgN)
 {
 	extern __shared__ double2 sh[];
-	int m = nx >> 1, lgm = 0;
-	while((1 << lgm) < m)
-		lgm++;
+	int p = find_plasma(pre, P, blockIdx.x);
+	const Geo &g = geo[p];
+	int nx = g.nx, ny = g.ny, R = g.R;
+	const double2 *__restrict__ ghat = ghat_all + g.ghat0;
+	double *__restrict__ phi = phi_all + g.node0;
+	int m = nx >> 1, lgm = g.lgnx - 1;
 	int stride = m + 1;
-	int y0 = blockIdx.x * R;
+	int y0 = (blockIdx.x - pre[p]) * R;
 
 	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
 	{
@@ -296,18 +312,28 @@ void k_rows_c2r(const double2 *__restrict__ ghat, double *__restrict__ phi,
 	}
 }
 
-// E = -grad(phi) (src/field.c:381-406) and zeroing of the next rho buffer.
+// E = -grad(phi) (src/field.c:381-406), zeroing of the next rho buffer and
+// of the next diagnostic slot.  Work items: blocks of EF_NODES nodes.
 template <typename T2>
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_efield(const double *__restrict__ phi, T2 *__restrict__ E, int nx, int ny,
-		double dx2, double dy2, double *__restrict__ rho_zero,
-		DiagSlot *__restrict__ diag_zero)
+void k_efield(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
+		const double *__restrict__ phi_all, T2 *__restrict__ E_all,
+		double *__restrict__ rho_zero, DiagSlot *__restrict__ diag, int ring,
+		const StepState *__restrict__ st)
 {
+	int p = find_plasma(pre, P, blockIdx.x);
+	const Geo &g = geo[p];
+	int nx = g.nx, ny = g.ny;
+	int blk = blockIdx.x - pre[p];
+	const double *__restrict__ phi = phi_all + g.node0;
+	T2 *__restrict__ E = E_all + g.node0;
+	double dx2 = 2 * g.dx, dy2 = 2 * g.dy;
 	int64_t n = (int64_t) nx * ny;
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-			i += (int64_t) gridDim.x * blockDim.x)
+	int64_t i0 = (int64_t) blk * EF_NODES;
+	int64_t i1 = i0 + EF_NODES < n ? i0 + EF_NODES : n;
+	for(int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
 	{
-		int iy = (int) (i / nx), ix = (int) (i - (int64_t) iy * nx);
+		int iy = (int) (i >> g.lgnx), ix = (int) (i & (nx - 1));
 		int x0 = ix == 0 ? nx - 1 : ix - 1, x1 = ix == nx - 1 ? 0 : ix + 1;
 		int y0 = iy == 0 ? ny - 1 : iy - 1, y1 = iy == ny - 1 ? 0 : iy + 1;
 		const double *row = phi + (int64_t) iy * nx;
@@ -318,14 +344,15 @@ void k_efield(const double *__restrict__ phi, T2 *__restrict__ E, int nx, int ny
 		v.y = ey;
 		E[i] = v;
 		if(rho_zero)
-			rho_zero[i] = 0.0;
+			rho_zero[g.node0 + i] = 0.0;
 	}
-	if(diag_zero && blockIdx.x == 0 && threadIdx.x == 0)
+	if(diag && blk == 0 && threadIdx.x == 0)
 	{
-		diag_zero->field = 0.0;
-		diag_zero->kinetic = 0.0;
-		diag_zero->px = 0.0;
-		diag_zero->py = 0.0;
+		DiagSlot *z = &diag[g.diag0 + (st->iter + 1) % ring];
+		z->field = 0.0;
+		z->kinetic = 0.0;
+		z->px = 0.0;
+		z->py = 0.0;
 	}
 }
 
@@ -361,15 +388,20 @@ twiddle(int64_t k, int64_t n, double *c, double *s)
 int
 cpic_fft_init(cpic_dev *d)
 {
-	int64_t nx = d->g.nx, ny = d->g.ny;
-	int64_t N = nx > ny ? nx : ny;
-	int64_t hx = nx / 2 + 1, k;
-	double cx = 2.0 * M_PI / nx, cy = 2.0 * M_PI / ny;
+	int64_t N = 1, k, ncos = 0;
 	double *ct;
 	double2 *tw;
+	int p;
 
+	for(p = 0; p < d->P; p++)
+	{
+		const Geo *g = &d->hgeo[p];
+		if(g->nx > N) N = g->nx;
+		if(g->ny > N) N = g->ny;
+		ncos += g->nx / 2 + 1 + g->ny;
+	}
 	d->lgN = ilog2(N);
-	ct = (double *) malloc(sizeof(double) * (size_t) (hx + ny));
+	ct = (double *) malloc(sizeof(double) * (size_t) ncos);
 	tw = (double2 *) malloc(sizeof(double2) * (size_t) (N / 2 + 1));
 	if(!ct || !tw)
 	{
@@ -377,20 +409,26 @@ cpic_fft_init(cpic_dev *d)
 		return CPIC_ENOMEM;
 	}
 	// MFT_init's cosines (src/solver.c:258-280), bit-identical
-	for(k = 0; k < hx; k++)
-		ct[k] = cos(cx * k);
-	for(k = 0; k < ny; k++)
-		ct[hx + k] = cos(cy * k);
+	for(p = 0; p < d->P; p++)
+	{
+		const Geo *g = &d->hgeo[p];
+		int64_t hx = g->nx / 2 + 1;
+		double cx = 2.0 * M_PI / g->nx, cy = 2.0 * M_PI / g->ny;
+		for(k = 0; k < hx; k++)
+			ct[g->cos0 + k] = cos(cx * k);
+		for(k = 0; k < g->ny; k++)
+			ct[g->cos0 + hx + k] = cos(cy * k);
+	}
 	for(k = 0; k <= N / 2; k++)
 		twiddle(k, N, &tw[k].x, &tw[k].y);
 
-	if(hipMalloc(&d->costab, sizeof(double) * (size_t) (hx + ny)) != hipSuccess ||
+	if(hipMalloc(&d->costab, sizeof(double) * (size_t) ncos) != hipSuccess ||
 			hipMalloc(&d->tw, sizeof(double2) * (size_t) (N / 2 + 1)) != hipSuccess)
 	{
 		free(ct); free(tw);
 		return CPIC_ENOMEM;
 	}
-	hipError_t e1 = hipMemcpy(d->costab, ct, sizeof(double) * (size_t) (hx + ny),
+	hipError_t e1 = hipMemcpy(d->costab, ct, sizeof(double) * (size_t) ncos,
 			hipMemcpyHostToDevice);
 	hipError_t e2 = hipMemcpy(d->tw, tw, sizeof(double2) * (size_t) (N / 2 + 1),
 			hipMemcpyHostToDevice);
@@ -408,93 +446,80 @@ cpic_fft_init(cpic_dev *d)
 	return 0;
 }
 
-static int
-rows_per_block(int64_t nx, int64_t ny)
+// FFT blocking of one plasma: R rows per row block (power of two dividing
+// ny, within the LDS budget, at most 8), C columns per column block.
+void
+cpic_fft_blocking(Geo *g)
 {
-	int64_t per = (nx / 2 + 1) * (int64_t) sizeof(double2);
-	int64_t R = FFT_LDS_BYTES / per;
-	int64_t want = 8;
-	if(R > want)
-		R = want;
-	if(R > ny)
-		R = ny;
-	// power of two dividing ny
-	int64_t r = 1;
+	int64_t per = (int64_t) (g->nx / 2 + 1) * (int64_t) sizeof(double2);
+	int64_t R = FFT_LDS_BYTES / per, r = 1;
+	if(R > 8)
+		R = 8;
+	if(R > g->ny)
+		R = g->ny;
 	while(r * 2 <= R)
 		r *= 2;
-	return (int) r;
+	g->R = (int) r;
+	int64_t C = FFT_LDS_BYTES / ((int64_t) g->ny * (int64_t) sizeof(double2));
+	if(C < 1)
+		C = 1;
+	if(C > g->nx / 2 + 1)
+		C = g->nx / 2 + 1;
+	if(C > 16)
+		C = 16;
+	g->C = (int) C;
 }
 
 int
-cpic_solve(cpic_dev *d, int64_t iter, int want_energy)
+cpic_solve(cpic_dev *d, int want_energy)
 {
-	int nx = d->g.nx, ny = d->g.ny, m = nx / 2;
-	int R = rows_per_block(nx, ny);
-	int C;
 	int slot;
-	size_t lds;
-	double *energy = NULL;
-	double N = (double) nx * (double) ny;
+	const int P = d->P;
+	const int *pre = d->pre;
+	DiagSlot *diag = (want_energy && !(d->flags & CPIC_F_NO_DIAG)) ? d->diag : (DiagSlot *) NULL;
 
-	// pass 1
-	lds = sizeof(double2) * (size_t) R * (size_t) (m + 1);
 	slot = cpic_kt_begin(d, "fft_rows_r2c");
-	hipLaunchKernelGGL(k_rows_r2c, dim3(ny / R), dim3(CPIC_BLOCK), lds, d->stream,
-			d->rho[d->rho_cur], d->ghat, nx, ny, R, d->tw, d->lgN);
+	hipLaunchKernelGGL(k_rows_r2c, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
+			d->geo, pre + PRE_ROW * (P + 1), P, d->rho[d->rho_cur], d->ghat, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
-	// pass 2
-	C = (int) (FFT_LDS_BYTES / ((int64_t) ny * (int64_t) sizeof(double2)));
-	if(C < 1)
-		C = 1;
-	if(C > m + 1)
-		C = m + 1;
-	if(C > 16)
-		C = 16;
-	lds = sizeof(double2) * (size_t) C * (size_t) ny;
-	if(want_energy && iter >= 0 && !(d->desc.flags & CPIC_F_NO_DIAG))
-		energy = &d->diag[iter % CPIC_DIAG_RING].field;
 	slot = cpic_kt_begin(d, "fft_cols_green");
-	hipLaunchKernelGGL(k_cols, dim3((m + 1 + C - 1) / C), dim3(CPIC_BLOCK), lds, d->stream,
-			d->ghat, nx, ny, C, d->costab, d->tw, d->lgN, 1.0 / N, energy,
-			-0.5 * d->desc.e0 / N);
+	hipLaunchKernelGGL(k_cols, dim3(d->colblocks), dim3(CPIC_BLOCK), d->lds_cols, d->stream,
+			d->geo, pre + PRE_COL * (P + 1), P, d->ghat, d->costab, d->tw, d->lgN, diag,
+			d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
-	// pass 3
-	lds = sizeof(double2) * (size_t) R * (size_t) (m + 1);
 	slot = cpic_kt_begin(d, "fft_rows_c2r");
-	hipLaunchKernelGGL(k_rows_c2r, dim3(ny / R), dim3(CPIC_BLOCK), lds, d->stream,
-			d->ghat, d->phi, nx, ny, R, d->tw, d->lgN);
+	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
+			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, d->phi, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
 
 int
-cpic_efield(cpic_dev *d, int64_t iter)
+cpic_efield(cpic_dev *d)
 {
-	int nx = d->g.nx, ny = d->g.ny;
-	int64_t n = (int64_t) nx * ny;
-	int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
 	double *rz = NULL;
 	DiagSlot *dz = NULL;
 	int slot;
+	const int P = d->P;
 
-	if(blocks > 8192)
-		blocks = 8192;
 	if(!d->rho_next_ready && !d->rho_pending)
 		rz = d->rho[1 - d->rho_cur];
-	if(!(d->desc.flags & CPIC_F_NO_DIAG))
-		dz = &d->diag[(iter + 1) % CPIC_DIAG_RING];
+	if(!(d->flags & CPIC_F_NO_DIAG))
+		dz = d->diag;
 	slot = cpic_kt_begin(d, "efield");
 	if(d->prec == 64)
-		hipLaunchKernelGGL(k_efield<double2>, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
-				d->phi, (double2 *) d->E, nx, ny, 2 * d->g.dx, 2 * d->g.dy, rz, dz);
+		hipLaunchKernelGGL(k_efield<double2>, dim3(d->nodeblocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->geo, d->pre + PRE_NODEBLK * (P + 1), P, d->phi, (double2 *) d->E, rz, dz,
+				d->ring, d->st);
 	else
-		hipLaunchKernelGGL(k_efield<float2>, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
-				d->phi, (float2 *) d->E, nx, ny, 2 * d->g.dx, 2 * d->g.dy, rz, dz);
+		hipLaunchKernelGGL(k_efield<float2>, dim3(d->nodeblocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->geo, d->pre + PRE_NODEBLK * (P + 1), P, d->phi, (float2 *) d->E, rz, dz,
+				d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	if(rz)

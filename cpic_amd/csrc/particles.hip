@@ -38,24 +38,51 @@ template <typename T> struct V2;
 template <> struct V2<double> { typedef double2 t; };
 template <> struct V2<float> { typedef float2 t; };
 
+// Arguments of the tile kernels.  Every block handles one tile of the batch
+// (tile_map of the block index), finds the tile's plasma in pre_tile and
+// takes the plasma's geometry and constants from geo.
 struct PushArgs {
-	Geo g;
-	Species sp;
+	const Geo *geo;
+	const int *pre_tile;  // [P+1] first global tile of each plasma
+	int P;
+	int ntiles;           // tiles of the whole batch
 	Buckets bk;
 	Parts p;
 	Leavers lv;
 	int par;              // leaver counter parity
-	const void *E;
-	double *rho;          // deposit target
-	DiagSlot *diag;       // nullable
+	int first;            // iteration 0: dt' = -dt/2 and no position update
+	const void *E;        // whole batch, plasma p at node0
+	double *rho;          // deposit target, whole batch
+	DiagSlot *diag;       // nullable: per-plasma rings of `ring` slots
+	int ring;
+	const StepState *st;
 	int *derr;
 	Counters *ctr;
-	int set_r;            // 0 on iteration 0 (no position update)
-	int mode;             // PUSH / DEPOSIT / GATHER
-	double dtp;           // dt' (dt, or -dt/2 on iteration 0)
-	double umax_x, umax_y;
-	float dtdx, dtdy;     // FP32 mode: dt' / dx, dt' / dy
 };
+
+// Per-species constants of the push, derived on the device from the
+// plasma's table entry with the host's expressions, so they are the same
+// doubles the reference computes: dtqm2 = 0.5 * dt' * q / m and the
+// rotation factors t, s = 2t/(1+t^2) (src/mover.c:23-83, 204-227).
+template <typename T> struct SpeciesK {
+	T h, tz, sz, vq;
+	double mh;
+};
+
+template <typename T>
+static __device__ __forceinline__ SpeciesK<T> species_k(const Geo &g, int s, double dtp)
+{
+	SpeciesK<T> k;
+	double h = 0.5 * dtp * g.q[s] / g.m[s];
+	double tz = g.Bz * h;
+	double sden = fma(tz, tz, 1.0);
+	k.h = (T) h;
+	k.tz = (T) tz;
+	k.sz = (T) (2.0 * tz / sden);
+	k.vq = (T) (-g.q[s] / g.e0);
+	k.mh = 0.5 * g.m[s];
+	return k;
+}
 
 enum { MODE_PUSH = 0, MODE_DEPOSIT = 1, MODE_GATHER = 2 };
 
@@ -327,11 +354,14 @@ static __device__ __forceinline__ void tile_flush(const Geo &g, int ox, int oy, 
 // One particle: gather, Boris push, move + wrap, deposit at the new
 // position; returns whether its cell left the tile (and the destination).
 template <typename T>
-static __device__ __forceinline__ bool push_one(const PushArgs &a, const Geo &g,
-		const typename V2<T>::t *sE, const typename V2<T>::t *__restrict__ E, double *sR,
-		int tile, int ox, int oy, int s, T h, T tz, T sz, T vq, double mh,
-		PState<T> &q, int &ndst, double &ke, double &mpx, double &mpy, bool &bad, bool &nan_)
+static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<T>::t *sE,
+		const typename V2<T>::t *__restrict__ E, double *sR, double *__restrict__ rho,
+		int tile, int ox, int oy, int s, const SpeciesK<T> &k, bool set_r, double dtp,
+		float dtdx, float dtdy, PState<T> &q, int &ndst, double &ke, double &mpx,
+		double &mpy, bool &bad, bool &nan_)
 {
+	const T h = k.h, tz = k.tz, sz = k.sz;
+	const double mh = k.mh;
 	CellPos c;
 	T rx, ry, ex, ey;
 	locate(g, ox, oy, q.x, q.y, c, rx, ry);
@@ -352,7 +382,7 @@ static __device__ __forceinline__ bool push_one(const PushArgs &a, const Geo &g,
 	T unx = fma(h, ex, vplx);
 	T uny = fma(h, ey, vply);
 	// check_velocity (src/mover.c:98-137)
-	if(fabs((double) unx) > a.umax_x || fabs((double) uny) > a.umax_y)
+	if(fabs((double) unx) > g.umax_x || fabs((double) uny) > g.umax_y)
 		bad = true;
 	double ux0 = (double) q.ux, uy0 = (double) q.uy;
 	double ux1 = (double) unx, uy1 = (double) uny;
@@ -361,19 +391,19 @@ static __device__ __forceinline__ bool push_one(const PushArgs &a, const Geo &g,
 	mpy += mh * (uy0 + uy1);
 	q.ux = unx;
 	q.uy = uny;
-	if(a.set_r)
+	if(set_r)
 	{
 		if(sizeof(T) == 8)
 		{
-			double nx_ = fma((double) unx, a.dtp, (double) q.x);
-			double ny_ = fma((double) uny, a.dtp, (double) q.y);
+			double nx_ = fma((double) unx, dtp, (double) q.x);
+			double ny_ = fma((double) uny, dtp, (double) q.y);
 			q.x = (T) wrap64(nx_, g.L0);
 			q.y = (T) wrap64(ny_, g.L1);
 		}
 		else
 		{
-			q.x = (T) fmaf((float) unx, a.dtdx, (float) q.x);
-			q.y = (T) fmaf((float) uny, a.dtdy, (float) q.y);
+			q.x = (T) fmaf((float) unx, dtdx, (float) q.x);
+			q.y = (T) fmaf((float) uny, dtdy, (float) q.y);
 		}
 	}
 	if(!(isfinite((double) q.x) && isfinite((double) q.y) &&
@@ -384,11 +414,11 @@ static __device__ __forceinline__ bool push_one(const PushArgs &a, const Geo &g,
 	}
 	CellPos c2;
 	locate(g, ox, oy, q.x, q.y, c2, rx, ry);
-	deposit<T>(g, sR, a.rho, c2, rx, ry, vq);
+	deposit<T>(g, sR, rho, c2, rx, ry, k.vq);
 	int ntile = (c2.iy >> g.lgty) * g.ntx + (c2.ix >> g.lgtx);
 	if(ntile == tile)
 		return false;
-	ndst = ntile * g.ns + s;
+	ndst = g.bucket0 + ntile * g.ns + s;
 	if(sizeof(T) == 4)
 	{
 		// re-express relative to the destination tile
@@ -416,16 +446,22 @@ void k_push(PushArgs a)
 	__shared__ int s_kcount;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
 
-	const Geo &g = a.g;
+	int gt = tile_map(blockIdx.x, a.ntiles);
+	int pl = find_plasma(a.pre_tile, a.P, gt);
+	const Geo &g = a.geo[pl];
 	T2 *sE = (T2 *) smem;
 	double *sR = (double *) (smem + sizeof(T2) * g.W * g.H);
 
-	int tile = tile_map(blockIdx.x, g.ntiles);
+	int tile = gt - g.tile0;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
 	int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	unsigned long long lt = (1ull << lane) - 1ull;
-	const T2 *__restrict__ E = (const T2 *) a.E;
+	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
+	double *__restrict__ rho = a.rho + g.node0;
+	const double dtp = a.first ? -g.dt / 2 : g.dt;
+	const bool set_r = !a.first;
+	const float dtdx = (float) (dtp / g.dx), dtdy = (float) (dtp / g.dy);
 
 	tile_setup<T>(g, ox, oy, E, sE, sR);
 	if(tid == 0)
@@ -438,15 +474,13 @@ void k_push(PushArgs a)
 
 	for(int s = 0; s < g.ns; s++)
 	{
-		const int b = tile * g.ns + s;
+		const int b = g.bucket0 + tile * g.ns + s;
 		const int64_t bstart = a.bk.start[b];
 		const int cap = a.bk.cap[b], res = a.bk.res[b];
 		const int n1 = min(a.bk.n1[b], cap - res);
 		const int ntot = n1 + a.bk.n2[b];
 		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
-		const T h = (T) a.sp.h[s], tz = (T) a.sp.tz[s], sz = (T) a.sp.sz[s];
-		const T vq = (T) a.sp.vq[s];
-		const double mh = 0.5 * a.sp.m[s];
+		const SpeciesK<T> sk = species_k<T>(g, s, dtp);
 		int w = 0;
 
 		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
@@ -469,8 +503,8 @@ void k_push(PushArgs a)
 				leave[u] = false;
 				ndst[u] = 0;
 				if(act[u])
-					leave[u] = push_one<T>(a, g, sE, E, sR, tile, ox, oy, s, h, tz, sz, vq, mh,
-							q[u], ndst[u], ke, mpx, mpy, bad, nan_);
+					leave[u] = push_one<T>(g, sE, E, sR, rho, tile, ox, oy, s, sk, set_r,
+							dtp, dtdx, dtdy, q[u], ndst[u], ke, mpx, mpy, bad, nan_);
 			}
 			// leavers: one atomic per wavefront
 #pragma unroll
@@ -496,7 +530,7 @@ void k_push(PushArgs a)
 							leave[u] = false;
 							if(sizeof(T) == 4)
 							{
-								int dt_ = ndst[u] / g.ns;
+								int dt_ = (ndst[u] - g.bucket0) / g.ns;
 								int dtx = (dt_ & (g.ntx - 1)) * g.tx - (ox + g.M);
 								int dty = (dt_ >> g.lgntx) * g.ty - (oy + g.M);
 								if(dtx > g.nx / 2) dtx -= g.nx;
@@ -560,9 +594,10 @@ void k_push(PushArgs a)
 	if(nan_)
 		atomicOr(a.derr, CPIC_DERR_NAN);
 
-	tile_flush<T>(g, ox, oy, sR, a.rho);
+	tile_flush<T>(g, ox, oy, sR, rho);
 
-	if(a.diag)
+	long long iter = a.diag ? a.st->iter : -1;
+	if(iter >= 0)
 	{
 		for(int off = 32; off > 0; off >>= 1)
 		{
@@ -586,9 +621,10 @@ void k_push(PushArgs a)
 				x += s_red[1][i];
 				y += s_red[2][i];
 			}
-			unsafeAtomicAdd(&a.diag->kinetic, k);
-			unsafeAtomicAdd(&a.diag->px, x);
-			unsafeAtomicAdd(&a.diag->py, y);
+			DiagSlot *ds = &a.diag[g.diag0 + iter % a.ring];
+			unsafeAtomicAdd(&ds->kinetic, k);
+			unsafeAtomicAdd(&ds->px, x);
+			unsafeAtomicAdd(&ds->py, y);
 		}
 	}
 }
@@ -600,28 +636,30 @@ __global__ __launch_bounds__(CPIC_BLOCK)
 void k_deposit(PushArgs a)
 {
 	extern __shared__ unsigned char smem[];
-	const Geo &g = a.g;
+	int gt = tile_map(blockIdx.x, a.ntiles);
+	const Geo &g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
 	double *sR = (double *) smem;
-	int tile = tile_map(blockIdx.x, g.ntiles);
+	double *__restrict__ rho = a.rho + g.node0;
+	int tile = gt - g.tile0;
 	int ox = (tile & (g.ntx - 1)) * g.tx - g.M, oy = (tile >> g.lgntx) * g.ty - g.M;
 	tile_setup<T>(g, ox, oy, (const typename V2<T>::t *) NULL, (typename V2<T>::t *) NULL, sR);
 	__syncthreads();
 	for(int s = 0; s < g.ns; s++)
 	{
-		int b = tile * g.ns + s;
+		int b = g.bucket0 + tile * g.ns + s;
 		int n1 = bucket_n1(a.bk, b), ntot = n1 + a.bk.n2[b];
-		T vq = (T) a.sp.vq[s];
+		T vq = (T) (-g.q[s] / g.e0);
 		for(int j = threadIdx.x; j < ntot; j += CPIC_BLOCK)
 		{
 			int64_t slot = bucket_slot(a.bk, b, n1, j);
 			CellPos c;
 			T rx, ry;
 			locate(g, ox, oy, ((const T *) a.p.x)[slot], ((const T *) a.p.y)[slot], c, rx, ry);
-			deposit<T>(g, sR, a.rho, c, rx, ry, vq);
+			deposit<T>(g, sR, rho, c, rx, ry, vq);
 		}
 	}
 	__syncthreads();
-	tile_flush<T>(g, ox, oy, sR, a.rho);
+	tile_flush<T>(g, ox, oy, sR, rho);
 }
 
 // interpolate_f2p_E (src/interpolate.c:350-404): gathered E stored per
@@ -632,16 +670,17 @@ void k_gather(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
 	extern __shared__ unsigned char smem[];
-	const Geo &g = a.g;
+	int gt = tile_map(blockIdx.x, a.ntiles);
+	const Geo &g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
 	T2 *sE = (T2 *) smem;
-	const T2 *__restrict__ E = (const T2 *) a.E;
-	int tile = tile_map(blockIdx.x, g.ntiles);
+	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
+	int tile = gt - g.tile0;
 	int ox = (tile & (g.ntx - 1)) * g.tx - g.M, oy = (tile >> g.lgntx) * g.ty - g.M;
 	tile_setup<T>(g, ox, oy, E, sE, (double *) NULL);
 	__syncthreads();
 	for(int s = 0; s < g.ns; s++)
 	{
-		int b = tile * g.ns + s;
+		int b = g.bucket0 + tile * g.ns + s;
 		int n1 = bucket_n1(a.bk, b), ntot = n1 + a.bk.n2[b];
 		for(int j = threadIdx.x; j < ntot; j += CPIC_BLOCK)
 		{
@@ -660,16 +699,21 @@ void k_gather(PushArgs a)
 #define PUSH_U 4
 
 // Places leavers in their destination bucket, or back in the reserved
-// slots of their source bucket when the destination is full.
+// slots of their source bucket when the destination is full.  The last
+// kernel of a step: it also advances the device step counter.
 template <typename T, bool IDS, bool PE>
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_append(Geo g, Buckets bk, Parts p, Leavers lv, int par, Counters *ctr)
+void k_append(const Geo *__restrict__ geo, const int *__restrict__ pre_bucket, int P,
+		Buckets bk, Parts p, Leavers lv, int par, Counters *ctr, StepState *st)
 {
 	int64_t n = lv.count[par];
 	if(n > lv.cap)
 		n = lv.cap;
 	if(blockIdx.x == 0 && threadIdx.x == 0)
+	{
 		lv.count[1 - par] = 0;
+		st->iter += 1;
+	}
 	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
 			i += (int64_t) gridDim.x * blockDim.x)
 	{
@@ -696,7 +740,8 @@ void k_append(Geo g, Buckets bk, Parts p, Leavers lv, int par, Counters *ctr)
 			at = bk.start[s] + (bk.cap[s] - bk.res[s]) + s2;
 			if(sizeof(T) == 4)
 			{
-				int td = d / g.ns, ts = s / g.ns;
+				const Geo &g = geo[find_plasma(pre_bucket, P, d)];
+				int td = (d - g.bucket0) / g.ns, ts = (s - g.bucket0) / g.ns;
 				int dtx = ((td & (g.ntx - 1)) - (ts & (g.ntx - 1))) * g.tx;
 				int dty = ((td >> g.lgntx) - (ts >> g.lgntx)) * g.ty;
 				if(dtx > g.nx / 2) dtx -= g.nx;
@@ -716,15 +761,36 @@ void k_append(Geo g, Buckets bk, Parts p, Leavers lv, int par, Counters *ctr)
 
 // ------------------------------------------------------------- rebuild ---
 
+// Segment of staged upload element i: segpre[k] <= i < segpre[k+1].
+static __device__ __forceinline__ int find_seg(const int64_t *__restrict__ segpre, int nseg,
+		int64_t i)
+{
+	int lo = 0, hi = nseg - 1;
+	while(lo < hi)
+	{
+		int mid = (lo + hi + 1) >> 1;
+		if(segpre[mid] <= i)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
 // Destination bucket of each particle in the current layout (species kept),
-// skipping species flagged in `skip`.
+// skipping the species a plasma's bit mask in `skip` flags (they are being
+// replaced by an upload).  One block per bucket of the batch.
 template <typename T>
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_hist_layout(Geo g, Buckets bk, Parts p, int skip, int *__restrict__ cnt)
+void k_hist_layout(const Geo *__restrict__ geo, const int *__restrict__ pre_bucket, int P,
+		Buckets bk, Parts p, const int *__restrict__ skip, int *__restrict__ cnt)
 {
 	int b = blockIdx.x;
-	int s = b % g.ns, tile = b / g.ns;
-	if(skip & (1 << s))
+	int pl = find_plasma(pre_bucket, P, b);
+	const Geo &g = geo[pl];
+	int lb = b - g.bucket0;
+	int s = lb % g.ns, tile = lb / g.ns;
+	if(skip[pl] & (1 << s))
 		return;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
@@ -736,22 +802,52 @@ void k_hist_layout(Geo g, Buckets bk, Parts p, int skip, int *__restrict__ cnt)
 		T rx, ry;
 		locate(g, ox, oy, ((const T *) p.x)[slot], ((const T *) p.y)[slot], c, rx, ry);
 		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
-		atomicAdd(&cnt[nt * g.ns + s], 1);
+		atomicAdd(&cnt[g.bucket0 + nt * g.ns + s], 1);
 	}
 }
 
-__global__ __launch_bounds__(CPIC_BLOCK)
-void k_hist_stage(Geo g, int s, int64_t n, const double *__restrict__ x,
-		const double *__restrict__ y, int *__restrict__ cnt)
+// Staged uploads: segment k holds species seg_s[k] of plasma seg_p[k], its
+// nk = segpre[k+1] - segpre[k] particles stored as x[nk], y[nk], ux[nk],
+// uy[nk] from buf + 4*segpre[k]; ids (nullable) at id + segpre[k].
+struct Staged {
+	const int64_t *segpre;
+	const int *seg_p, *seg_s;
+	int nseg;
+	int64_t n;
+	const double *buf;
+	const int64_t *id;
+};
+
+struct StagedAt {
+	int seg;
+	int64_t nk, o;       // segment length, offset in the segment
+	const double *b;     // segment base
+};
+
+static __device__ __forceinline__ StagedAt staged_at(const Staged &u, int64_t i)
 {
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+	StagedAt r;
+	r.seg = find_seg(u.segpre, u.nseg, i);
+	int64_t s0 = u.segpre[r.seg];
+	r.nk = u.segpre[r.seg + 1] - s0;
+	r.o = i - s0;
+	r.b = u.buf + 4 * s0;
+	return r;
+}
+
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_hist_stage(const Geo *__restrict__ geo, Staged u, int *__restrict__ cnt)
+{
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < u.n;
 			i += (int64_t) gridDim.x * blockDim.x)
 	{
+		StagedAt a = staged_at(u, i);
+		const Geo &g = geo[u.seg_p[a.seg]];
 		CellPos c;
 		double rx, ry;
-		locate(g, 0, 0, x[i], y[i], c, rx, ry);
+		locate(g, 0, 0, a.b[a.o], a.b[a.nk + a.o], c, rx, ry);
 		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
-		atomicAdd(&cnt[nt * g.ns + s], 1);
+		atomicAdd(&cnt[g.bucket0 + nt * g.ns + u.seg_s[a.seg]], 1);
 	}
 }
 
@@ -799,12 +895,16 @@ void k_scan_caps(const int *__restrict__ cnt, int nb, double slack, int pad, int
 
 template <typename T, bool IDS, bool PE>
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_scatter_layout(Geo g, Buckets bk, Parts p, int skip, const int64_t *__restrict__ nstart,
+void k_scatter_layout(const Geo *__restrict__ geo, const int *__restrict__ pre_bucket, int P,
+		Buckets bk, Parts p, const int *__restrict__ skip, const int64_t *__restrict__ nstart,
 		int *__restrict__ fill, Parts np)
 {
 	int b = blockIdx.x;
-	int s = b % g.ns, tile = b / g.ns;
-	if(skip & (1 << s))
+	int pl = find_plasma(pre_bucket, P, b);
+	const Geo &g = geo[pl];
+	int lb = b - g.bucket0;
+	int s = lb % g.ns, tile = lb / g.ns;
+	if(skip[pl] & (1 << s))
 		return;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
@@ -823,7 +923,7 @@ void k_scatter_layout(Geo g, Buckets bk, Parts p, int skip, const int64_t *__res
 		T rx, ry;
 		locate(g, ox, oy, q.x, q.y, c, rx, ry);
 		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
-		int nb_ = nt * g.ns + s;
+		int nb_ = g.bucket0 + nt * g.ns + s;
 		if(sizeof(T) == 4)
 		{
 			q.x = (T) ((float) (c.ix & (g.tx - 1)) + (float) rx);
@@ -836,33 +936,35 @@ void k_scatter_layout(Geo g, Buckets bk, Parts p, int skip, const int64_t *__res
 
 template <typename T, bool IDS, bool PE>
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_scatter_stage(Geo g, int s, int64_t n, const double *__restrict__ x,
-		const double *__restrict__ y, const double *__restrict__ ux,
-		const double *__restrict__ uy, const int64_t *__restrict__ id,
-		const int64_t *__restrict__ nstart, int *__restrict__ fill, Parts np)
+void k_scatter_stage(const Geo *__restrict__ geo, Staged u, const int64_t *__restrict__ nstart,
+		int *__restrict__ fill, Parts np)
 {
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < u.n;
 			i += (int64_t) gridDim.x * blockDim.x)
 	{
+		StagedAt a = staged_at(u, i);
+		const Geo &g = geo[u.seg_p[a.seg]];
+		int s = u.seg_s[a.seg];
+		double xi = a.b[a.o], yi = a.b[a.nk + a.o];
 		CellPos c;
 		double rx, ry;
-		locate(g, 0, 0, x[i], y[i], c, rx, ry);
+		locate(g, 0, 0, xi, yi, c, rx, ry);
 		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
-		int nb_ = nt * g.ns + s;
+		int nb_ = g.bucket0 + nt * g.ns + s;
 		PState<T> q;
 		if(sizeof(T) == 8)
 		{
-			q.x = (T) x[i];
-			q.y = (T) y[i];
+			q.x = (T) xi;
+			q.y = (T) yi;
 		}
 		else
 		{
 			q.x = (T) ((float) (c.ix & (g.tx - 1)) + (float) rx);
 			q.y = (T) ((float) (c.iy & (g.ty - 1)) + (float) ry);
 		}
-		q.ux = (T) ux[i];
-		q.uy = (T) uy[i];
-		q.id = IDS ? id[i] : 0;
+		q.ux = (T) a.b[2 * a.nk + a.o];
+		q.uy = (T) a.b[3 * a.nk + a.o];
+		q.id = IDS && u.id ? u.id[i] : 0;
 		q.Ex = q.Ey = 0.0;
 		int k = atomicAdd(&fill[nb_], 1);
 		store_p<T>(np, nstart[nb_] + k, q, IDS, PE);
@@ -872,11 +974,11 @@ void k_scatter_stage(Geo g, int s, int64_t n, const double *__restrict__ x,
 // ------------------------------------------------------------ download ---
 
 __global__ __launch_bounds__(CPIC_BLOCK)
-void k_species_counts(Buckets bk, int ns, int s, int ntiles, int *__restrict__ cnt)
+void k_species_counts(Buckets bk, int bucket0, int ns, int s, int ntiles, int *__restrict__ cnt)
 {
 	for(int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
 	{
-		int b = t * ns + s;
+		int b = bucket0 + t * ns + s;
 		cnt[t] = bucket_n1(bk, b) + bk.n2[b];
 	}
 }
@@ -888,7 +990,7 @@ void k_download(Geo g, Buckets bk, Parts p, int s, const int64_t *__restrict__ o
 		int64_t *__restrict__ ocell, int64_t n)
 {
 	int tile = blockIdx.x;
-	int b = tile * g.ns + s;
+	int b = g.bucket0 + tile * g.ns + s;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
 	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
@@ -927,7 +1029,7 @@ __global__ __launch_bounds__(CPIC_BLOCK)
 void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__restrict__ counts)
 {
 	int tile = blockIdx.x;
-	int b = tile * g.ns + s;
+	int b = g.bucket0 + tile * g.ns + s;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
 	int n1 = bucket_n1(bk, b), ntot = n1 + bk.n2[b];
@@ -943,9 +1045,10 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 
 // ---------------------------------------------------------------- host ---
 
+
 #define DISPATCH(kern, T, ...) do { \
-		bool ids_ = (d->desc.flags & CPIC_F_IDS) != 0; \
-		bool pe_ = (d->desc.flags & CPIC_F_PARTICLE_E) != 0; \
+		bool ids_ = (d->flags & CPIC_F_IDS) != 0; \
+		bool pe_ = (d->flags & CPIC_F_PARTICLE_E) != 0; \
 		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true>), __VA_ARGS__); \
 		else if(ids_) hipLaunchKernelGGL((kern<T, true, false>), __VA_ARGS__); \
 		else if(pe_) hipLaunchKernelGGL((kern<T, false, true>), __VA_ARGS__); \
@@ -953,8 +1056,8 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 	} while(0)
 
 #define DISPATCH_U(kern, T, ...) do { \
-		bool ids_ = (d->desc.flags & CPIC_F_IDS) != 0; \
-		bool pe_ = (d->desc.flags & CPIC_F_PARTICLE_E) != 0; \
+		bool ids_ = (d->flags & CPIC_F_IDS) != 0; \
+		bool pe_ = (d->flags & CPIC_F_PARTICLE_E) != 0; \
 		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, PUSH_U>), __VA_ARGS__); \
 		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, PUSH_U>), __VA_ARGS__); \
 		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, PUSH_U>), __VA_ARGS__); \
@@ -966,10 +1069,34 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 		else DISPATCH(kern, float, __VA_ARGS__); \
 	} while(0)
 
+#define PRE(d, k) ((d)->pre + (k) * ((d)->P + 1))
+
 static size_t
 elem(const cpic_dev *d)
 {
 	return d->prec == 64 ? sizeof(double) : sizeof(float);
+}
+
+// largest LDS tile region of the batch, in nodes
+static size_t
+max_wh(const cpic_dev *d)
+{
+	size_t m = 0;
+	for(int p = 0; p < d->P; p++)
+	{
+		size_t n = (size_t) d->hgeo[p].W * d->hgeo[p].H;
+		if(n > m)
+			m = n;
+	}
+	return m;
+}
+
+size_t
+cpic_push_lds(const cpic_dev *d)
+{
+	size_t n = max_wh(d);
+	return d->prec == 64 ? n * (sizeof(double2) + sizeof(double))
+		: n * (sizeof(float2) + sizeof(double));
 }
 
 void
@@ -994,9 +1121,9 @@ cpic_parts_alloc(cpic_dev *d, int which, int64_t cap)
 			hipMalloc(&p->ux, e * cap) != hipSuccess ||
 			hipMalloc(&p->uy, e * cap) != hipSuccess)
 		goto fail;
-	if((d->desc.flags & CPIC_F_IDS) && hipMalloc(&p->id, sizeof(int64_t) * cap) != hipSuccess)
+	if((d->flags & CPIC_F_IDS) && hipMalloc(&p->id, sizeof(int64_t) * cap) != hipSuccess)
 		goto fail;
-	if((d->desc.flags & CPIC_F_PARTICLE_E) &&
+	if((d->flags & CPIC_F_PARTICLE_E) &&
 			(hipMalloc(&p->Ex, sizeof(double) * cap) != hipSuccess ||
 			 hipMalloc(&p->Ey, sizeof(double) * cap) != hipSuccess))
 		goto fail;
@@ -1008,50 +1135,43 @@ fail:
 }
 
 static PushArgs
-push_args(cpic_dev *d, int64_t iter, int mode, double *rho)
+push_args(cpic_dev *d, int first, double *rho)
 {
 	PushArgs a;
 	memset(&a, 0, sizeof(a));
-	a.g = d->g;
+	a.geo = d->geo;
+	a.pre_tile = PRE(d, PRE_TILE);
+	a.P = d->P;
+	a.ntiles = d->ntiles;
 	a.bk = d->bk;
 	a.p = d->p[d->pcur];
 	a.lv = d->lv;
 	a.par = d->step_parity;
+	a.first = first;
 	a.E = d->E;
 	a.rho = rho;
+	a.diag = (d->flags & CPIC_F_NO_DIAG) ? (DiagSlot *) NULL : d->diag;
+	a.ring = d->ring;
+	a.st = d->st;
 	a.derr = d->derr;
 	a.ctr = d->ctr;
-	a.mode = mode;
-	double dtp = iter == 0 ? -d->dt / 2 : d->dt;
-	a.set_r = iter == 0 ? 0 : 1;
-	a.dtp = dtp;
-	a.dtdx = (float) (dtp / d->g.dx);
-	a.dtdy = (float) (dtp / d->g.dy);
-	a.umax_x = d->umax[0];
-	a.umax_y = d->umax[1];
-	for(int s = 0; s < d->ns; s++)
-	{
-		// dtqm2 = 0.5 * dt * q / m (src/mover.c:211,222); t, s per component
-		double h = 0.5 * dtp * d->desc.q[s] / d->desc.m[s];
-		double tz = d->desc.B[2] * h;
-		double sden = fma(tz, tz, 1.0);
-		a.sp.h[s] = h;
-		a.sp.tz[s] = tz;
-		a.sp.sz[s] = 2.0 * tz / sden;
-		a.sp.vq[s] = -d->desc.q[s] / d->desc.e0;
-		a.sp.m[s] = d->desc.m[s];
-	}
-	if(mode == MODE_PUSH && iter >= 0 && !(d->desc.flags & CPIC_F_NO_DIAG))
-		a.diag = &d->diag[iter % CPIC_DIAG_RING];
 	return a;
 }
 
-static size_t
-push_lds(const cpic_dev *d)
+__global__ void k_set_iter(StepState *st, long long iter)
 {
-	size_t n = (size_t) d->g.W * d->g.H;
-	return d->prec == 64 ? n * (sizeof(double2) + sizeof(double))
-		: n * (sizeof(float2) + sizeof(double));
+	if(blockIdx.x == 0 && threadIdx.x == 0)
+		st->iter = iter;
+}
+
+// Sets the device step counter with a kernel (also valid inside a capture).
+int
+cpic_set_iter(cpic_dev *d, int64_t iter)
+{
+	hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(64), 0, d->stream, d->st, (long long) iter);
+	HIPCHK(hipGetLastError());
+	d->iter = iter;
+	return 0;
 }
 
 int
@@ -1068,39 +1188,39 @@ cpic_zero(cpic_dev *d, double *p, int64_t n)
 }
 
 int
-cpic_parts_push(cpic_dev *d, int64_t iter)
+cpic_parts_push(cpic_dev *d, int first_iter)
 {
-	PushArgs a = push_args(d, iter, MODE_PUSH, d->rho[1 - d->rho_cur]);
+	PushArgs a = push_args(d, first_iter, d->rho[1 - d->rho_cur]);
 	int slot;
-	size_t lds = push_lds(d);
 
 	slot = cpic_kt_begin(d, "push_gather_deposit");
 	if(d->prec == 64)
-		DISPATCH_U(k_push, double, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		DISPATCH_U(k_push, double, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	else
-		DISPATCH_U(k_push, float, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		DISPATCH_U(k_push, float, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
 	slot = cpic_kt_begin(d, "append_leavers");
-	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->g, d->bk,
-			d->p[d->pcur], d->lv, d->step_parity, d->ctr);
+	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->geo, PRE(d, PRE_BUCKET),
+			d->P, d->bk, d->p[d->pcur], d->lv, d->step_parity, d->ctr, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	d->step_parity ^= 1;
+	d->iter++;
 	return 0;
 }
 
 int
 cpic_parts_deposit(cpic_dev *d, double *rho_out)
 {
-	PushArgs a = push_args(d, 1, MODE_DEPOSIT, rho_out);
+	PushArgs a = push_args(d, 0, rho_out);
 	int slot = cpic_kt_begin(d, "deposit");
-	size_t lds = (size_t) d->g.W * d->g.H * sizeof(double);
+	size_t lds = max_wh(d) * sizeof(double);
 	if(d->prec == 64)
-		hipLaunchKernelGGL(k_deposit<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		hipLaunchKernelGGL(k_deposit<double>, dim3(d->ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	else
-		hipLaunchKernelGGL(k_deposit<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		hipLaunchKernelGGL(k_deposit<float>, dim3(d->ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1109,87 +1229,118 @@ cpic_parts_deposit(cpic_dev *d, double *rho_out)
 int
 cpic_parts_gather(cpic_dev *d)
 {
-	PushArgs a = push_args(d, 1, MODE_GATHER, d->rho[d->rho_cur]);
+	PushArgs a = push_args(d, 0, d->rho[d->rho_cur]);
 	int slot = cpic_kt_begin(d, "gather");
-	size_t lds = (size_t) d->g.W * d->g.H * (d->prec == 64 ? sizeof(double2) : sizeof(float2));
+	size_t lds = max_wh(d) * (d->prec == 64 ? sizeof(double2) : sizeof(float2));
 	if(d->prec == 64)
-		hipLaunchKernelGGL(k_gather<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		hipLaunchKernelGGL(k_gather<double>, dim3(d->ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	else
-		hipLaunchKernelGGL(k_gather<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
+		hipLaunchKernelGGL(k_gather<float>, dim3(d->ntiles), dim3(CPIC_BLOCK), lds, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
 
-// Full re-sort: counts per destination bucket (current layout + pending
-// uploads), fresh capacities, scatter into the spare particle arrays.
+// Full re-sort of the batch: counts per destination bucket (current layout
+// + pending uploads), fresh capacities, scatter into the spare particle
+// arrays.  Pending uploads replace their (plasma, species) wholesale.
 int
 cpic_parts_rebuild(cpic_dev *d)
 {
-	int nb = d->nb, skip = 0, s;
-	int *cnt = NULL, *fill = NULL;
+	const int P = d->P, MS = CPIC_MAX_SPECIES;
+	int nb = d->nb, p, s, k;
+	int *cnt = NULL, *fill = NULL, *ncap = NULL, *dskip = NULL;
 	int64_t *nstart = NULL, *total = NULL, tot = 0;
 	double *stage = NULL;
-	int64_t *sid = NULL;
-	int ret = CPIC_ENOMEM;
-	int *ncap = NULL;
+	int64_t *sid = NULL, *dsegpre = NULL;
+	int *dseg = NULL;
+	int ret = CPIC_ENOMEM, nseg = 0;
+	int *skip = (int *) calloc((size_t) P, sizeof(int));
+	int64_t *segpre = (int64_t *) malloc(sizeof(int64_t) * (size_t) (P * MS + 1));
+	int *seg = (int *) malloc(sizeof(int) * 2 * (size_t) (P * MS));
+	Staged u;
 
-	for(s = 0; s < d->ns; s++)
-		if(d->up_buf[s])
-			skip |= 1 << s;
+	if(!skip || !segpre || !seg)
+		goto out;
+	segpre[0] = 0;
+	for(p = 0; p < P; p++)
+		for(s = 0; s < MS; s++)
+		{
+			if(!d->up_buf[p * MS + s])
+				continue;
+			skip[p] |= 1 << s;
+			if(d->up_n[p * MS + s] == 0)
+				continue;
+			seg[nseg] = p;
+			seg[P * MS + nseg] = s;
+			segpre[nseg + 1] = segpre[nseg] + d->up_n[p * MS + s];
+			nseg++;
+		}
 
 	HIPCHK(hipStreamSynchronize(d->stream));
 	if(hipMalloc(&cnt, sizeof(int) * nb) != hipSuccess ||
 			hipMalloc(&fill, sizeof(int) * nb) != hipSuccess ||
 			hipMalloc(&ncap, sizeof(int) * nb) != hipSuccess ||
 			hipMalloc(&nstart, sizeof(int64_t) * nb) != hipSuccess ||
-			hipMalloc(&total, sizeof(int64_t)) != hipSuccess)
+			hipMalloc(&total, sizeof(int64_t)) != hipSuccess ||
+			hipMalloc(&dskip, sizeof(int) * P) != hipSuccess)
 		goto out;
-	hipMemsetAsync(cnt, 0, sizeof(int) * nb, d->stream);
-	hipMemsetAsync(fill, 0, sizeof(int) * nb, d->stream);
+	HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int) * nb, d->stream));
+	HIPCHK(hipMemsetAsync(fill, 0, sizeof(int) * nb, d->stream));
+	HIPCHK(hipMemcpyAsync(dskip, skip, sizeof(int) * P, hipMemcpyHostToDevice, d->stream));
 
 	if(d->pcap[d->pcur] > 0)
 	{
 		if(d->prec == 64)
 			hipLaunchKernelGGL(k_hist_layout<double>, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
-					d->g, d->bk, d->p[d->pcur], skip, cnt);
+					d->geo, PRE(d, PRE_BUCKET), P, d->bk, d->p[d->pcur], dskip, cnt);
 		else
 			hipLaunchKernelGGL(k_hist_layout<float>, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
-					d->g, d->bk, d->p[d->pcur], skip, cnt);
+					d->geo, PRE(d, PRE_BUCKET), P, d->bk, d->p[d->pcur], dskip, cnt);
 	}
-	// stage uploads
+	memset(&u, 0, sizeof(u));
+	if(nseg > 0)
 	{
-		int64_t maxn = 0;
-		for(s = 0; s < d->ns; s++)
-			if(d->up_buf[s] && d->up_n[s] > maxn)
-				maxn = d->up_n[s];
-		if(maxn > 0)
+		int64_t n = segpre[nseg];
+		if(hipMalloc(&stage, sizeof(double) * 4 * n) != hipSuccess ||
+				hipMalloc(&dsegpre, sizeof(int64_t) * (nseg + 1)) != hipSuccess ||
+				hipMalloc(&dseg, sizeof(int) * 2 * nseg) != hipSuccess)
+			goto out;
+		if((d->flags & CPIC_F_IDS) && hipMalloc(&sid, sizeof(int64_t) * n) != hipSuccess)
+			goto out;
+		HIPCHK(hipMemcpyAsync(dsegpre, segpre, sizeof(int64_t) * (nseg + 1),
+				hipMemcpyHostToDevice, d->stream));
+		HIPCHK(hipMemcpyAsync(dseg, seg, sizeof(int) * nseg, hipMemcpyHostToDevice, d->stream));
+		HIPCHK(hipMemcpyAsync(dseg + nseg, seg + P * MS, sizeof(int) * nseg,
+				hipMemcpyHostToDevice, d->stream));
+		for(k = 0; k < nseg; k++)
 		{
-			if(hipMalloc(&stage, sizeof(double) * 4 * maxn * d->ns) != hipSuccess ||
-					hipMalloc(&sid, sizeof(int64_t) * maxn * d->ns) != hipSuccess)
-				goto out;
+			int i = seg[k] * MS + seg[P * MS + k];
+			int64_t nk = segpre[k + 1] - segpre[k];
+			HIPCHK(hipMemcpyAsync(stage + 4 * segpre[k], d->up_buf[i], sizeof(double) * 4 * nk,
+					hipMemcpyHostToDevice, d->stream));
+			if(sid && d->up_id[i])
+				HIPCHK(hipMemcpyAsync(sid + segpre[k], d->up_id[i], sizeof(int64_t) * nk,
+						hipMemcpyHostToDevice, d->stream));
 		}
-		for(s = 0; s < d->ns; s++)
-		{
-			int64_t n = d->up_n[s];
-			double *st = stage + (int64_t) 4 * maxn * s;
-			if(!d->up_buf[s] || n == 0)
-				continue;
-			hipMemcpyAsync(st, d->up_buf[s], sizeof(double) * 4 * n, hipMemcpyHostToDevice,
-					d->stream);
-			if(d->up_id[s])
-				hipMemcpyAsync(sid + maxn * s, d->up_id[s], sizeof(int64_t) * n,
-						hipMemcpyHostToDevice, d->stream);
-			int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
-			if(blocks > 4096) blocks = 4096;
-			hipLaunchKernelGGL(k_hist_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
-					d->g, s, n, st, st + n, cnt);
-		}
+		u.segpre = dsegpre;
+		u.seg_p = dseg;
+		u.seg_s = dseg + nseg;
+		u.nseg = nseg;
+		u.n = n;
+		u.buf = stage;
+		u.id = sid;
+		int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+		if(blocks > 8192) blocks = 8192;
+		hipLaunchKernelGGL(k_hist_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream, d->geo,
+				u, cnt);
+	}
+	{
 		// slack > 0: spare fraction; 0: default; < 0: no spare at all (tight
 		// capacities, which exercises the fallback path)
-		double slack = d->desc.slack > 0 ? d->desc.slack : 0.25;
+		double slack = d->desc[0].slack > 0 ? d->desc[0].slack : 0.25;
 		int pad = 64;
-		if(d->desc.slack < 0)
+		if(d->desc[0].slack < 0)
 		{
 			slack = 0.0;
 			pad = 0;
@@ -1206,19 +1357,14 @@ cpic_parts_rebuild(cpic_dev *d)
 				goto out;
 		if(d->pcap[d->pcur] > 0)
 			DISPATCH_P(k_scatter_layout, dim3(nb), dim3(CPIC_BLOCK), 0, d->stream,
-					d->g, d->bk, d->p[d->pcur], skip, nstart, fill, d->p[spare]);
-		for(s = 0; s < d->ns; s++)
-		{
-			int64_t n = d->up_n[s];
-			double *st = stage + (int64_t) 4 * maxn * s;
-			if(!d->up_buf[s] || n == 0)
-				continue;
-			int blocks = (int) ((n + CPIC_BLOCK - 1) / CPIC_BLOCK);
-			if(blocks > 4096) blocks = 4096;
-			DISPATCH_P(k_scatter_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
-					d->g, s, n, st, st + n, st + 2 * n, st + 3 * n,
-					d->up_id[s] ? sid + maxn * s : (const int64_t *) NULL, nstart, fill,
+					d->geo, PRE(d, PRE_BUCKET), P, d->bk, d->p[d->pcur], dskip, nstart, fill,
 					d->p[spare]);
+		if(nseg > 0)
+		{
+			int blocks = (int) ((u.n + CPIC_BLOCK - 1) / CPIC_BLOCK);
+			if(blocks > 8192) blocks = 8192;
+			DISPATCH_P(k_scatter_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
+					d->geo, u, nstart, fill, d->p[spare]);
 		}
 		HIPCHK(hipGetLastError());
 		// new metadata: n1 = count, res = n2 = 0
@@ -1230,31 +1376,34 @@ cpic_parts_rebuild(cpic_dev *d)
 		HIPCHK(hipStreamSynchronize(d->stream));
 		d->pcur = spare;
 	}
-	for(s = 0; s < d->ns; s++)
+	for(k = 0; k < P * MS; k++)
 	{
-		if(d->up_buf[s])
+		if(d->up_buf[k])
 		{
-			d->count[s] = d->up_n[s];
-			free(d->up_buf[s]);
-			free(d->up_id[s]);
-			d->up_buf[s] = NULL;
-			d->up_id[s] = NULL;
-			d->up_n[s] = 0;
+			d->count[k] = d->up_n[k];
+			free(d->up_buf[k]);
+			free(d->up_id[k]);
+			d->up_buf[k] = NULL;
+			d->up_id[k] = NULL;
+			d->up_n[k] = 0;
 		}
 	}
 	d->pending = 0;
 	d->nrebuilds++;
+	d->graph_ok = 0;         // captured steps name the old particle arrays
 	ret = 0;
 out:
 	hipFree(cnt); hipFree(fill); hipFree(ncap); hipFree(nstart); hipFree(total);
-	hipFree(stage); hipFree(sid);
+	hipFree(dskip); hipFree(stage); hipFree(sid); hipFree(dsegpre); hipFree(dseg);
+	free(skip); free(segpre); free(seg);
 	return ret;
 }
 
 int
-cpic_parts_download(cpic_dev *d, int s, cpic_particles_t *out)
+cpic_parts_download(cpic_dev *d, int plasma, int s, cpic_particles_t *out)
 {
-	int nt = d->g.ntiles;
+	const Geo &g = d->hgeo[plasma];
+	int nt = g.ntiles;
 	int *cnt = NULL;
 	int64_t *off = NULL, *tot = NULL, n = 0;
 	double *buf = NULL;
@@ -1268,7 +1417,7 @@ cpic_parts_download(cpic_dev *d, int s, cpic_particles_t *out)
 			hipMalloc(&tot, sizeof(int64_t)) != hipSuccess)
 		goto out;
 	hipLaunchKernelGGL(k_species_counts, dim3((nt + CPIC_BLOCK - 1) / CPIC_BLOCK), dim3(CPIC_BLOCK),
-			0, d->stream, d->bk, d->ns, s, nt, cnt);
+			0, d->stream, d->bk, g.bucket0, g.ns, s, nt, cnt);
 	hipLaunchKernelGGL(k_scan_caps, dim3(1), dim3(1024), 0, d->stream, cnt, nt, 0.0, 0, 1,
 			off, (int *) NULL, tot);
 	HIPCHK(hipMemcpyAsync(&n, tot, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
@@ -1286,7 +1435,7 @@ cpic_parts_download(cpic_dev *d, int s, cpic_particles_t *out)
 				hipMalloc(&oid, sizeof(int64_t) * n) != hipSuccess ||
 				hipMalloc(&ocell, sizeof(int64_t) * n) != hipSuccess)
 			goto out;
-		DISPATCH_P(k_download, dim3(nt), dim3(CPIC_BLOCK), 0, d->stream, d->g, d->bk,
+		DISPATCH_P(k_download, dim3(nt), dim3(CPIC_BLOCK), 0, d->stream, g, d->bk,
 				d->p[d->pcur], s, off, buf, oid, ocell, n);
 		HIPCHK(hipGetLastError());
 		h = (double *) malloc(sizeof(double) * 6 * n);
@@ -1316,19 +1465,20 @@ out:
 }
 
 int
-cpic_parts_cell_counts(cpic_dev *d, int s, int64_t *counts)
+cpic_parts_cell_counts(cpic_dev *d, int plasma, int s, int64_t *counts)
 {
+	const Geo &g = d->hgeo[plasma];
 	unsigned long long *c = NULL;
-	int64_t n = (int64_t) d->g.nx * d->g.ny;
+	int64_t n = (int64_t) g.nx * g.ny;
 	if(hipMalloc(&c, sizeof(unsigned long long) * n) != hipSuccess)
 		return CPIC_ENOMEM;
 	hipMemsetAsync(c, 0, sizeof(unsigned long long) * n, d->stream);
 	if(d->prec == 64)
-		hipLaunchKernelGGL(k_cell_counts<double>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
-				d->g, d->bk, d->p[d->pcur], s, c);
+		hipLaunchKernelGGL(k_cell_counts<double>, dim3(g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
+				g, d->bk, d->p[d->pcur], s, c);
 	else
-		hipLaunchKernelGGL(k_cell_counts<float>, dim3(d->g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
-				d->g, d->bk, d->p[d->pcur], s, c);
+		hipLaunchKernelGGL(k_cell_counts<float>, dim3(g.ntiles), dim3(CPIC_BLOCK), 0, d->stream,
+				g, d->bk, d->p[d->pcur], s, c);
 	hipError_t e = hipMemcpyAsync(counts, c, sizeof(int64_t) * n, hipMemcpyDeviceToHost, d->stream);
 	hipStreamSynchronize(d->stream);
 	hipFree(c);
@@ -1345,6 +1495,8 @@ cpic_parts_init(cpic_dev *d)
 		(const void *) k_push<double, false, true, PUSH_U>, (const void *) k_push<double, true, true, PUSH_U>,
 		(const void *) k_push<float, false, false, PUSH_U>, (const void *) k_push<float, true, false, PUSH_U>,
 		(const void *) k_push<float, false, true, PUSH_U>, (const void *) k_push<float, true, true, PUSH_U>,
+		(const void *) k_deposit<double>, (const void *) k_deposit<float>,
+		(const void *) k_gather<double>, (const void *) k_gather<float>,
 	};
 	for(unsigned i = 0; i < sizeof(k) / sizeof(k[0]); i++)
 		HIPCHK(hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize,

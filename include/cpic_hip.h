@@ -41,6 +41,8 @@ extern "C" {
 #define CPIC_F_PARTICLE_E  0x2   /* keep the gathered E per particle (ppack.E, src/def.h:96) */
 #define CPIC_F_PROFILE     0x4   /* time every kernel with HIP events */
 #define CPIC_F_NO_DIAG     0x8   /* skip the energy/momentum diagnostics */
+#define CPIC_F_NO_GRAPH    0x10  /* cpic_dev_run launches every kernel itself
+                                    instead of replaying captured steps */
 
 /* fields for cpic_dev_download_field / cpic_dev_upload_field */
 #define CPIC_RHO 0
@@ -140,7 +142,10 @@ int cpic_dev_field_rho(cpic_dev_t *dev, int64_t iter);
 int cpic_dev_step(cpic_dev_t *dev, int64_t iter);
 
 /* nsteps calls of cpic_dev_step starting at iter0, queued without host
- * synchronisation. */
+ * synchronisation (sim_run's loop, src/sim.c:594-617, without the per-step
+ * host work).  Steps >= 1 are replayed two at a time from a HIP graph
+ * captured on first use (not with CPIC_F_PROFILE or CPIC_F_NO_GRAPH); the
+ * results are the same as those of cpic_dev_step. */
 int cpic_dev_run(cpic_dev_t *dev, int64_t iter0, int64_t nsteps);
 
 /* Waits for queued work and returns any latched device error. */
@@ -183,6 +188,38 @@ int cpic_dev_stats(cpic_dev_t *dev, cpic_dev_stats_t *st);
 int cpic_dev_kernel_times(cpic_dev_t *dev, int32_t *n, const char **names,
 		int64_t *launches, double *total_ms);
 int cpic_dev_reset_kernel_times(cpic_dev_t *dev);
+
+/*
+ * Batches.  One cpic_dev_t may hold P independent plasmas (an ensemble of
+ * runs, e.g. a parameter scan of the reference's configurations): each has
+ * its own grid, species, constants and particles, and every stage call runs
+ * all of them with one launch per kernel.  The reference runs one plasma per
+ * process; these calls have no counterpart there and extend the per-plasma
+ * calls above with a plasma index.  cpic_dev_create is a batch of one, and
+ * the per-plasma cpic_dev_* calls address plasma 0.  The stage calls
+ * (pre_step, field_E, plasma_E, plasma_r, field_rho, step, run, solve_xy,
+ * sync, rebuild) and cpic_dev_migration / stats / kernel_times act on the
+ * whole batch; migration reports batch-wide tile indices (plasma p's tiles
+ * follow those of plasmas 0..p-1).
+ */
+
+/* sim_init's device part for nplasmas configurations at once.  They must
+ * share precision, flags, device and slack. */
+int cpic_batch_create(const cpic_dev_desc_t *descs, int32_t nplasmas, cpic_dev_t **out);
+int cpic_batch_size(cpic_dev_t *dev, int32_t *nplasmas);
+int cpic_batch_upload_particles(cpic_dev_t *dev, int32_t plasma, int32_t species,
+		int64_t n, const int64_t *id, const double *x, const double *y,
+		const double *ux, const double *uy);
+int cpic_batch_download_particles(cpic_dev_t *dev, int32_t plasma, int32_t species,
+		cpic_particles_t *out);
+int cpic_batch_count(cpic_dev_t *dev, int32_t plasma, int32_t species, int64_t *n);
+int cpic_batch_download_field(cpic_dev_t *dev, int32_t plasma, int32_t which, double *out);
+int cpic_batch_upload_field(cpic_dev_t *dev, int32_t plasma, int32_t which,
+		const double *in);
+int cpic_batch_energy(cpic_dev_t *dev, int32_t plasma, int64_t max, double *field,
+		double *kinetic, double *px, double *py, int64_t *n);
+int cpic_batch_cell_counts(cpic_dev_t *dev, int32_t plasma, int32_t species,
+		int64_t *counts);
 
 #ifdef __cplusplus
 }

@@ -43,3 +43,17 @@ def test_invalid_descriptions_rejected_before_any_device_call():
         with pytest.raises(ca.CpicError) as ei:
             ca.Device(**args)
         assert ei.value.code == -1, kw
+
+
+def test_invalid_batches_rejected_before_any_device_call():
+    good = dict(nx=16, ny=16, L=(16.0, 16.0), dt=0.1, q=[-1.0], m=[1.0])
+    mixed = [dict(precision=32), dict(flags=ca.F_IDS), dict(slack=0.5), dict(nx=48)]
+    for kw in mixed:
+        other = dict(good)
+        other.update(kw)
+        with pytest.raises(ca.CpicError) as ei:
+            ca.Batch([ca.make_desc(**good), ca.make_desc(**other)])
+        assert ei.value.code == -1, kw
+    with pytest.raises(ca.CpicError) as ei:
+        ca.Batch([])
+    assert ei.value.code == -1
