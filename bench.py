@@ -12,6 +12,12 @@ comparison.  Inputs are generated on the host (synthetic, seeded), uploaded,
 and resident in HBM before the timed region.
 
 Prints one JSON line.  `value` = particle-steps/s of the FP32 step.
+
+`--workload ensemble` instead times a batch of 256 independent two-stream
+plasmas in one handle (grids drawn from 32^2..256^2, 16-128 particles per
+cell, v0 swept 0.05-0.2 and sigma 0.005-0.02, FP32 particles + FP64 FFT
+Poisson, tile shapes from the autotuner), 300 steps replayed from captured
+HIP graphs, and prints its own JSON line.
 """
 import argparse
 import json
@@ -168,6 +174,92 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     log(f"copy calibration: {reps} x {nbytes} bytes device to device")
 
 
+ENSEMBLE_GRIDS = (32, 64, 128, 256)
+
+
+def ensemble_workload(nplasmas=256, seed=256):
+    """Per plasma: grid edge, particles per cell, drift v0 and spread sigma
+    (unit cells, L = grid, m = ppc so that w_p = 1, dt = 0.1)."""
+    rng = np.random.default_rng(seed)
+    grids = rng.choice(ENSEMBLE_GRIDS, nplasmas)
+    ppcs = rng.integers(16, 129, nplasmas)
+    v0 = np.linspace(0.05, 0.2, nplasmas)
+    sigma = np.linspace(0.005, 0.02, nplasmas)
+    return [dict(grid=int(g), ppc=int(c), v0=float(v), sigma=float(s), dt=0.1)
+            for g, c, v, s in zip(grids, ppcs, v0, sigma)]
+
+
+def run_ensemble(args, ca):
+    from cpic_amd import autotune
+    plist = ensemble_workload(args.plasmas)
+    table = autotune.load()
+    missing = sorted({p["grid"] for p in plist if f"fp32/{p['grid']}" not in table})
+    if missing and args.autotune:
+        table = autotune.tune(missing, 32, log=log)
+    descs = []
+    for p in plist:
+        g = p["grid"]
+        descs.append(ca.make_desc(g, g, (float(g), float(g)), p["dt"], [-1.0, -1.0],
+                                  [float(p["ppc"]), float(p["ppc"])], precision=32,
+                                  tile=autotune.tile_for(g, 32, table), flags=0))
+    t0 = time.time()
+    b = ca.Batch(descs)
+    empty = np.empty(0, np.int64)
+    ntot = 0
+    for i, p in enumerate(plist):
+        rng = np.random.default_rng([args.seed, i])
+        parts, _ = autotune.two_stream(rng, p["grid"], p["ppc"], p["v0"], p["sigma"])
+        for s, (x, y, ux, uy) in enumerate(parts):
+            b.upload(s, empty, x, y, ux, uy, plasma=i)
+            ntot += len(x)
+        if i % 32 == 31:
+            log(f"ensemble: generated {i + 1} plasmas, {ntot} particles")
+    b.pre_step()
+    b.run(0, args.warmup)
+    b.sync()
+    log(f"ensemble: {len(plist)} plasmas, {ntot} particles, init {time.time() - t0:.1f} s")
+    steps = args.ens_steps - args.warmup
+    t0 = time.perf_counter()
+    b.run(args.warmup, steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    # per-kernel times over a few more steps (launched one by one)
+    b.set_profiling(True)
+    b.reset_kernel_times()
+    b.run(args.ens_steps, 10)
+    b.sync()
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    nodes = sum(p["grid"] ** 2 for p in plist)
+    push_n, push_ms = kt.get("push_gather_deposit", (0, 0.0))
+    avg_push = push_ms / max(push_n, 1) / 1e3
+    alg = ntot * 32 + nodes * 24
+    achieved = alg / avg_push / 1e9 if avg_push > 0 else None
+    value = ntot * steps / el
+    grids = {g: sum(1 for p in plist if p["grid"] == g) for g in ENSEMBLE_GRIDS}
+    out = {
+        "metric": "particle-steps/s, batched ensemble of independent two-stream plasmas",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded two-stream electrons, host-generated)",
+        "config": {"workload": f"{len(plist)} two-stream plasmas in one batch, grids "
+                               f"{dict((f'{g}^2', n) for g, n in grids.items())}, 16-128 ppc, "
+                               f"{ntot} particles, FP32 particles + FP64 FFT Poisson, "
+                               f"captured-graph replay",
+                   "plasmas": len(plist), "particles": ntot, "nodes": nodes,
+                   "tiles": {str(g): list(autotune.tile_for(g, 32, table)) for g in ENSEMBLE_GRIDS},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
+                     "kernel": "push_gather_deposit", "alg_bytes_per_launch": alg,
+                     "avg_launch_ms": avg_push * 1e3},
+        "kernels_ms": {k: ms / max(n, 1) for k, (n, ms) in kt.items()},
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +274,12 @@ def main():
                     help="steps over which FP32 and FP64 field energies are compared (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 1 GiB device-to-device copies (PMC byte-counter calibration)")
+    ap.add_argument("--workload", choices=("twostream", "ensemble"), default="twostream")
+    ap.add_argument("--plasmas", type=int, default=256, help="ensemble: number of plasmas")
+    ap.add_argument("--ens-steps", type=int, default=300, help="ensemble: steps incl. warmup")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--autotune", action="store_true",
+                    help="ensemble: tune tile shapes of grids missing from the cache first")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -191,6 +289,9 @@ def main():
         sys.exit(2)
 
     import cpic_amd as ca
+    if args.workload == "ensemble":
+        run_ensemble(args, ca)
+        return
     if args.calibrate_copy:
         copy_calibration()
     w = workload(args.grid, args.ppc)
@@ -206,17 +307,24 @@ def main():
         log(f"fp{prec}: init {time.time() - t0:.1f} s")
         el, kt = time_steps(d, 0, args.warmup, steps)
         st = d.stats()
+        # the same steps replayed from captured graphs, without kernel timers
+        d.set_profiling(False)
+        t0 = time.perf_counter()
+        d.run(args.warmup + steps, steps)
+        d.sync()
+        el_graph = time.perf_counter() - t0
         log(f"fp{prec}: {steps} steps in {el:.3f} s, {el / steps * 1e3:.3f} ms/step, {st}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
             log(f"   {k:24s} {n:6d} launches {ms / max(n, 1):9.4f} ms avg")
         hist = None
         if args.accuracy_steps > 0:
-            done = args.warmup + steps
+            done = args.warmup + 2 * steps
             if args.accuracy_steps > done:
                 d.run(done, args.accuracy_steps - done)   # untimed, same seed in both precisions
             d.sync()
             hist = d.energy()["field"][:args.accuracy_steps]
-        res[prec] = dict(elapsed=el, steps=steps, kt=kt, stats=st, hist=hist)
+        res[prec] = dict(elapsed=el, elapsed_graph=el_graph, steps=steps, kt=kt, stats=st,
+                         hist=hist)
         d.close()
         del d
 
@@ -235,6 +343,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": r32["elapsed"] / r32["steps"] * 1e3,
+        "ms_per_step_graph_replay": r32["elapsed_graph"] / r32["steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -27,10 +27,11 @@ EXPORTS = (
     "cpic_dev_run", "cpic_dev_sync", "cpic_dev_solve_xy", "cpic_solver_rho_size",
     "cpic_dev_download_field", "cpic_dev_upload_field", "cpic_dev_energy", "cpic_dev_migration",
     "cpic_dev_cell_counts", "cpic_dev_rebuild", "cpic_dev_stats", "cpic_dev_kernel_times",
-    "cpic_dev_reset_kernel_times", "cpic_batch_create", "cpic_batch_size",
+    "cpic_dev_reset_kernel_times", "cpic_dev_set_profiling", "cpic_batch_create", "cpic_batch_size",
     "cpic_batch_upload_particles", "cpic_batch_download_particles", "cpic_batch_count",
     "cpic_batch_download_field", "cpic_batch_upload_field", "cpic_batch_energy",
-    "cpic_batch_cell_counts",
+    "cpic_batch_cell_counts", "cpic_adj_create", "cpic_adj_destroy", "cpic_adj_forward",
+    "cpic_adj_backward", "cpic_adj_state",
 )
 
 
@@ -103,6 +104,7 @@ def lib():
         L.cpic_dev_stats.argtypes = [vp, P(Stats)]
         L.cpic_dev_kernel_times.argtypes = [vp, i32, P(C.c_char_p), i64, d]
         L.cpic_dev_reset_kernel_times.argtypes = [vp]
+        L.cpic_dev_set_profiling.argtypes = [vp, C.c_int32]
         L.cpic_batch_create.argtypes = [P(Desc), C.c_int32, P(vp)]
         L.cpic_batch_size.argtypes = [vp, i32]
         L.cpic_batch_upload_particles.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, i64,
@@ -113,6 +115,11 @@ def lib():
         L.cpic_batch_upload_field.argtypes = [vp, C.c_int32, C.c_int32, d]
         L.cpic_batch_energy.argtypes = [vp, C.c_int32, C.c_int64, d, d, d, d, i64]
         L.cpic_batch_cell_counts.argtypes = [vp, C.c_int32, C.c_int32, i64]
+        L.cpic_adj_create.argtypes = [P(Desc), C.c_int64, i32, P(vp)]
+        L.cpic_adj_destroy.argtypes = [vp]
+        L.cpic_adj_forward.argtypes = [vp, d, d, d, d, C.c_int64, C.c_int32, d, d]
+        L.cpic_adj_backward.argtypes = [vp, C.c_double, d, d, d, d]
+        L.cpic_adj_state.argtypes = [vp, d, d, d, d]
         _lib = L
     return _lib
 
@@ -302,6 +309,9 @@ class Device:
 
     def reset_kernel_times(self):
         check(lib().cpic_dev_reset_kernel_times(self._h))
+
+    def set_profiling(self, on):
+        check(lib().cpic_dev_set_profiling(self._h, 1 if on else 0))
 
 
 class Batch(Device):

@@ -1145,6 +1145,21 @@ cpic_dev_kernel_times(cpic_dev_t *d, int32_t *n, const char **names, int64_t *la
 }
 
 extern "C" int
+cpic_dev_set_profiling(cpic_dev_t *d, int32_t on)
+{
+	if(!d)
+		return CPIC_EINVAL;
+	if(on)
+		d->flags |= CPIC_F_PROFILE;
+	else
+	{
+		kt_collect(d);
+		d->flags &= ~CPIC_F_PROFILE;
+	}
+	return 0;
+}
+
+extern "C" int
 cpic_dev_reset_kernel_times(cpic_dev_t *d)
 {
 	if(!d)

@@ -198,6 +198,8 @@ void cpic_fft_blocking(Geo *g);
 size_t cpic_push_lds(const cpic_dev *d);
 int cpic_set_iter(cpic_dev *d, int64_t iter);
 int cpic_solve(cpic_dev *d, int want_energy);
+int cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi);
+int cpic_efield_ptr(cpic_dev *d, const double *phi, void *E);
 int cpic_efield(cpic_dev *d);
 int cpic_parts_init(cpic_dev *d);
 int cpic_parts_rebuild(cpic_dev *d);

@@ -107,7 +107,7 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 {
 	extern __shared__ double2 sh[];
 	int p = find_plasma(pre, P, blockIdx.x);
-	const Geo &g = geo[p];
+	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, R = g.R;
 	const double *__restrict__ rho = rho_all + g.node0;
 	double2 *__restrict__ ghat = ghat_all + g.ghat0;
@@ -178,7 +178,7 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	extern __shared__ double2 sh[];
 	__shared__ double red[CPIC_BLOCK];
 	int p = find_plasma(pre, P, blockIdx.x);
-	const Geo &g = geo[p];
+	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, C = g.C;
 	double2 *__restrict__ ghat = ghat_all + g.ghat0;
 	int m = nx >> 1;
@@ -247,7 +247,7 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 {
 	extern __shared__ double2 sh[];
 	int p = find_plasma(pre, P, blockIdx.x);
-	const Geo &g = geo[p];
+	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, R = g.R;
 	const double2 *__restrict__ ghat = ghat_all + g.ghat0;
 	double *__restrict__ phi = phi_all + g.node0;
@@ -322,7 +322,7 @@ void k_efield(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const StepState *__restrict__ st)
 {
 	int p = find_plasma(pre, P, blockIdx.x);
-	const Geo &g = geo[p];
+	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny;
 	int blk = blockIdx.x - pre[p];
 	const double *__restrict__ phi = phi_all + g.node0;
@@ -470,17 +470,19 @@ cpic_fft_blocking(Geo *g)
 	g->C = (int) C;
 }
 
-int
-cpic_solve(cpic_dev *d, int want_energy)
+// phi = S rho for every plasma of the batch (rho and phi laid out like the
+// state's own buffers); with diag, the field energy of each plasma goes to
+// its slot of the current step.
+static int
+solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag)
 {
 	int slot;
 	const int P = d->P;
 	const int *pre = d->pre;
-	DiagSlot *diag = (want_energy && !(d->flags & CPIC_F_NO_DIAG)) ? d->diag : (DiagSlot *) NULL;
 
 	slot = cpic_kt_begin(d, "fft_rows_r2c");
 	hipLaunchKernelGGL(k_rows_r2c, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
-			d->geo, pre + PRE_ROW * (P + 1), P, d->rho[d->rho_cur], d->ghat, d->tw, d->lgN);
+			d->geo, pre + PRE_ROW * (P + 1), P, rho, d->ghat, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
@@ -493,8 +495,37 @@ cpic_solve(cpic_dev *d, int want_energy)
 
 	slot = cpic_kt_begin(d, "fft_rows_c2r");
 	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
-			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, d->phi, d->tw, d->lgN);
+			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, phi, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int
+cpic_solve(cpic_dev *d, int want_energy)
+{
+	DiagSlot *diag = (want_energy && !(d->flags & CPIC_F_NO_DIAG)) ? d->diag : (DiagSlot *) NULL;
+	return solve(d, d->rho[d->rho_cur], d->phi, diag);
+}
+
+int
+cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi)
+{
+	return solve(d, rho, phi, (DiagSlot *) NULL);
+}
+
+int
+cpic_efield_ptr(cpic_dev *d, const double *phi, void *E)
+{
+	const int P = d->P;
+	if(d->prec == 64)
+		hipLaunchKernelGGL(k_efield<double2>, dim3(d->nodeblocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->geo, d->pre + PRE_NODEBLK * (P + 1), P, phi, (double2 *) E,
+				(double *) NULL, (DiagSlot *) NULL, d->ring, d->st);
+	else
+		hipLaunchKernelGGL(k_efield<float2>, dim3(d->nodeblocks), dim3(CPIC_BLOCK), 0, d->stream,
+				d->geo, d->pre + PRE_NODEBLK * (P + 1), P, phi, (float2 *) E,
+				(double *) NULL, (DiagSlot *) NULL, d->ring, d->st);
 	HIPCHK(hipGetLastError());
 	return 0;
 }

@@ -29,6 +29,7 @@
 // (u_z stays 0).  The translation unit is compiled with -ffp-contract=off;
 // the multiply-adds the reference's clang build contracts are explicit fma().
 #include "dev.h"
+#include "picmath.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -113,49 +114,6 @@ static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
 {
 	int64_t s = bk.start[b];
 	return j < n1 ? s + j : s + (bk.cap[b] - bk.res[b]) + (j - n1);
-}
-
-// FP64 cell index and relative offset (src/interpolate.c:39-66).  The Y
-// middle term uses dx (the reference passes dx[X], src/interpolate.c:88).
-static __device__ __forceinline__ void cell64(const Geo &g, double x, double y,
-		int &ix, int &iy, double &rx, double &ry)
-{
-	double bx = x * g.idx;
-	double sx = floor(bx);
-	double gx = fma(-sx, g.dx, x);
-	rx = gx * g.idx;
-	ix = (int) sx;
-	if(ix >= g.nx)
-		ix -= g.nx;
-	double by = y * g.idy;
-	double sy = floor(by);
-	double gy = fma(-sy, g.dx, y);
-	ry = gy * g.idy;
-	iy = (int) sy;
-	if(iy >= g.ny)
-		iy -= g.ny;
-}
-
-// periodic_boundary_ppack with the x == L -> 0 rule.  A displacement larger
-// than L (only possible after |u| > umax, which is flagged) is reduced with
-// floor so that every index stays inside the grid.
-static __device__ __forceinline__ double wrap64(double r, double L)
-{
-	if(r >= L)
-		r -= L;
-	else if(r < 0.0)
-	{
-		r += L;
-		if(r >= L)
-			r = 0.0;
-	}
-	if(!(r >= 0.0 && r < L))
-	{
-		r = r - floor(r / L) * L;
-		if(!(r >= 0.0 && r < L))
-			r = 0.0;
-	}
-	return r;
 }
 
 // Per-particle state in registers.
@@ -368,19 +326,8 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 	gather<T>(g, sE, E, c, rx, ry, ex, ey);
 	q.Ex = (double) ex;
 	q.Ey = (double) ey;
-	// boris_rotation reduced to B = (0,0,Bz)
-	T vmx = fma(h, ex, q.ux);
-	T vmy = fma(h, ey, q.uy);
-	T t1 = vmy * tz;
-	T vpx = t1 + vmx;
-	T t2 = vmx * tz;
-	T vpy = vmy - t2;
-	T t3 = vpy * sz;
-	T vplx = t3 + vmx;
-	T t4 = vpx * sz;
-	T vply = vmy - t4;
-	T unx = fma(h, ex, vplx);
-	T uny = fma(h, ey, vply);
+	T unx = q.ux, uny = q.uy;
+	boris_bz<T>(h, tz, sz, ex, ey, unx, uny);
 	// check_velocity (src/mover.c:98-137)
 	if(fabs((double) unx) > g.umax_x || fabs((double) uny) > g.umax_y)
 		bad = true;
@@ -448,7 +395,11 @@ void k_push(PushArgs a)
 
 	int gt = tile_map(blockIdx.x, a.ntiles);
 	int pl = find_plasma(a.pre_tile, a.P, gt);
-	const Geo &g = a.geo[pl];
+	// the plasma's table entry, read once into registers (a reference into
+	// global memory would be re-read after every particle store); the
+	// per-species arrays are read through gp
+	const Geo *gp = &a.geo[pl];
+	const Geo g = *gp;
 	T2 *sE = (T2 *) smem;
 	double *sR = (double *) (smem + sizeof(T2) * g.W * g.H);
 
@@ -480,7 +431,7 @@ void k_push(PushArgs a)
 		const int n1 = min(a.bk.n1[b], cap - res);
 		const int ntot = n1 + a.bk.n2[b];
 		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
-		const SpeciesK<T> sk = species_k<T>(g, s, dtp);
+		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 		int w = 0;
 
 		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
@@ -637,7 +588,8 @@ void k_deposit(PushArgs a)
 {
 	extern __shared__ unsigned char smem[];
 	int gt = tile_map(blockIdx.x, a.ntiles);
-	const Geo &g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
+	const Geo *gp = &a.geo[find_plasma(a.pre_tile, a.P, gt)];
+	const Geo g = *gp;
 	double *sR = (double *) smem;
 	double *__restrict__ rho = a.rho + g.node0;
 	int tile = gt - g.tile0;
@@ -648,7 +600,7 @@ void k_deposit(PushArgs a)
 	{
 		int b = g.bucket0 + tile * g.ns + s;
 		int n1 = bucket_n1(a.bk, b), ntot = n1 + a.bk.n2[b];
-		T vq = (T) (-g.q[s] / g.e0);
+		T vq = (T) (-gp->q[s] / g.e0);
 		for(int j = threadIdx.x; j < ntot; j += CPIC_BLOCK)
 		{
 			int64_t slot = bucket_slot(a.bk, b, n1, j);
@@ -671,7 +623,7 @@ void k_gather(PushArgs a)
 	typedef typename V2<T>::t T2;
 	extern __shared__ unsigned char smem[];
 	int gt = tile_map(blockIdx.x, a.ntiles);
-	const Geo &g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
+	const Geo g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
 	T2 *sE = (T2 *) smem;
 	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
 	int tile = gt - g.tile0;
@@ -787,7 +739,7 @@ void k_hist_layout(const Geo *__restrict__ geo, const int *__restrict__ pre_buck
 {
 	int b = blockIdx.x;
 	int pl = find_plasma(pre_bucket, P, b);
-	const Geo &g = geo[pl];
+	const Geo g = geo[pl];
 	int lb = b - g.bucket0;
 	int s = lb % g.ns, tile = lb / g.ns;
 	if(skip[pl] & (1 << s))
@@ -901,7 +853,7 @@ void k_scatter_layout(const Geo *__restrict__ geo, const int *__restrict__ pre_b
 {
 	int b = blockIdx.x;
 	int pl = find_plasma(pre_bucket, P, b);
-	const Geo &g = geo[pl];
+	const Geo g = geo[pl];
 	int lb = b - g.bucket0;
 	int s = lb % g.ns, tile = lb / g.ns;
 	if(skip[pl] & (1 << s))

@@ -7,6 +7,7 @@
  */
 #define _GNU_SOURCE
 #include "sim.h"
+#include "output.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -502,6 +503,13 @@ sim_init(const cfg_t *conf, int quiet)
 	if(sim_read_config(s, conf))
 		goto out;
 	cfg_get_bool(cfg_lookup(conf, "simulation.track_particles"), &track);
+	/* output_init (src/sim.c:268-273) */
+	s->output = calloc(1, sizeof(*s->output));
+	if(!s->output || output_init(s, s->output))
+	{
+		err("output_init failed\n");
+		goto out;
+	}
 
 	memset(&desc, 0, sizeof(desc));
 	desc.nx = s->ntpoints[X];
@@ -636,6 +644,11 @@ sim_step(sim_t *s)
 		cpic_dev_sync(s->dev);
 		perf_stop(&s->timers[TIMER_SOLVER]);
 	}
+	if(output_fields(s))
+	{
+		err("output_fields failed\n");
+		return -1;
+	}
 	stage_plasma_E(s);
 	stage_plasma_r(s);
 	stage_field_rho(s);
@@ -690,6 +703,8 @@ sim_free(sim_t *s)
 		return;
 	if(s->dev)
 		cpic_dev_destroy(s->dev);
+	output_free(s->output);
+	free(s->output);
 	free(s);
 }
 

@@ -35,6 +35,7 @@ enum timers {
 	TIMER_PARTICLE_X,
 	TIMER_TOTAL,
 	TIMER_ITERATION,
+	TIMER_OUTPUT_FIELDS,
 	MAX_TIMERS
 };
 
@@ -85,6 +86,7 @@ typedef struct sim {
 
 	perf_t timers[MAX_TIMERS];
 	cpic_dev_t *dev;
+	struct output *output;   /* output.c; field output (output.path) */
 } sim_t;
 
 /* sim_init (src/sim.c:239-320): reads the configuration, generates the

@@ -189,6 +189,12 @@ int cpic_dev_kernel_times(cpic_dev_t *dev, int32_t *n, const char **names,
 		int64_t *launches, double *total_ms);
 int cpic_dev_reset_kernel_times(cpic_dev_t *dev);
 
+/* Turns kernel timing (CPIC_F_PROFILE) on or off for later calls, like the
+ * reference's per-stage perf timers (src/perf.c, src/sim.c:287-293,
+ * 492-563).  Timed steps are launched one kernel at a time (no graph
+ * replay); times recorded so far are kept. */
+int cpic_dev_set_profiling(cpic_dev_t *dev, int32_t on);
+
 /*
  * Batches.  One cpic_dev_t may hold P independent plasmas (an ensemble of
  * runs, e.g. a parameter scan of the reference's configurations): each has
@@ -220,6 +226,38 @@ int cpic_batch_energy(cpic_dev_t *dev, int32_t plasma, int64_t max, double *fiel
 		double *kinetic, double *px, double *py, int64_t *n);
 int cpic_batch_cell_counts(cpic_dev_t *dev, int32_t plasma, int32_t species,
 		int64_t *counts);
+
+/*
+ * Differentiable runs (FP64).  The reference has no gradients; these calls
+ * run its loop (sim_step with the iteration-0 half step) on n particles
+ * that keep their index, and return reverse-mode gradients of the
+ * time-integrated field energy  loss = sum_{it < nsteps} W(it),
+ * W = -(e0/2) sum rho*phi  (the value cpic_dev_energy reports per step),
+ * with respect to every particle's initial position and velocity.  Forward
+ * states are checkpointed every `checkpoint_every` iterations and the
+ * reverse sweep recomputes each segment from its checkpoint.  desc must
+ * have precision 64; flags, tile and slack are ignored.
+ */
+typedef struct cpic_adj cpic_adj_t;
+
+/* species[i] (host, n entries) is the species of particle i. */
+int cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
+		cpic_adj_t **out);
+int cpic_adj_destroy(cpic_adj_t *adj);
+
+/* Runs iterations 0..nsteps-1 from the initial state (host arrays of n):
+ * *loss = sum of W; energy (nullable, nsteps entries) = W per iteration. */
+int cpic_adj_forward(cpic_adj_t *adj, const double *x, const double *y,
+		const double *ux, const double *uy, int64_t nsteps,
+		int32_t checkpoint_every, double *loss, double *energy);
+
+/* Gradients of dloss * loss with respect to the initial x, y, ux, uy of the
+ * last forward (host arrays of n; any may be NULL). */
+int cpic_adj_backward(cpic_adj_t *adj, double dloss, double *gx, double *gy,
+		double *gux, double *guy);
+
+/* State after the last forward (host arrays of n; any may be NULL). */
+int cpic_adj_state(cpic_adj_t *adj, double *x, double *y, double *ux, double *uy);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,8 @@ import tempfile
 import numpy as np
 import pytest
 
+from common import rel_l2
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "cpic_amd", "lib", "libcpic_host.so")
 BIN = os.path.join(ROOT, "cpic_amd", "bin", "cpic")
@@ -118,3 +120,71 @@ def test_cpic_cli_bad_conf():
     r = subprocess.run([BIN, os.path.join(CONF, "mpi.conf")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode != 0 and 'pblock_nmax' in r.stderr
+
+
+def test_field_output_reference_format():
+    """output.path: bin/<iter>/{rho,phi,E_X,E_Y}.bin (padded FP64 matrices,
+    aligned) and xdmf/fields-iter<iter>.xdmf whose HyperSlabs select the
+    interior (src/output.c:400-635), every output.period.field iterations;
+    the interiors equal the oracle's fields of that iteration."""
+    import xml.etree.ElementTree as ET
+    import pyoracle as po
+    from test_host import generate
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "out")
+        src = os.path.join(ROOT, "cpic_amd", "conf", "two-stream-64.conf")
+        text = open(src).read().replace("cycles = 200", "cycles = 5")
+        text += (f'\noutput = {{\n\tpath = "{out}"\n\tslices = 4\n\talignment = 4096\n'
+                 f'\tperiod = {{ field = 2 }}\n}}\n')
+        path = os.path.join(d, "c.conf")
+        with open(path, "w") as f:
+            f.write(text)
+        r = subprocess.run([BIN, "-q", path], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        o = po.Oracle(64, 64, (64.0, 64.0), 0.1, [-1.0, -1.0], [2.5, 2.5], nchunks=4)
+        for s in range(2):
+            ret, x, y, ux, uy = generate(path, s, 5120)
+            assert ret == 0
+            o.add(s, np.arange(5120, dtype=np.int64), x, y, ux, uy)
+        o.pre_step()
+        names = {"rho": "rho", "phi": "phi", "E_X": "Ex", "E_Y": "Ey"}
+        shapes = {"rho": (65, 66, 0), "phi": (67, 66, 1), "E_X": (65, 64, 0), "E_Y": (65, 64, 0)}
+        for it in range(5):
+            o.stage("field_E")
+            xd = os.path.join(out, "xdmf", f"fields-iter{it}.xdmf")
+            if it % 2:
+                assert not os.path.exists(xd)
+            else:
+                root = ET.parse(xd).getroot()
+                grid = root.find("Domain/Grid")
+                assert grid.find("Topology").get("NumberOfElements") == "1 64 64"
+                geo = grid.findall("Geometry/DataItem")
+                assert geo[1].text.split() == ["0.000000", "1.000000", "1.000000"]
+                attrs = grid.findall("Attribute")
+                assert [a.get("Name") for a in attrs] == ["phi", "rho", "E_X", "E_Y"]
+                for a in attrs:
+                    name = a.get("Name")
+                    rows, cols, dy = shapes[name]
+                    slab = a.find("DataItem")
+                    assert slab.get("Dimensions") == "1 64 64"
+                    sel, data = slab.findall("DataItem")
+                    assert sel.text.split()[:3] == ["0", str(dy), "0"]
+                    assert data.get("Dimensions") == f"1 {rows} {cols}"
+                    assert data.text.strip() == f"../bin/{it}/{name}.bin"
+                    fn = os.path.join(out, "bin", str(it), f"{name}.bin")
+                    nbytes = rows * cols * 8
+                    assert os.path.getsize(fn) == (nbytes + 4095) // 4096 * 4096
+                    m = np.fromfile(fn, dtype=np.float64, count=rows * cols).reshape(rows, cols)
+                    interior = m[dy:dy + 64, :64]
+                    ref = o.field(names[name])
+                    assert rel_l2(interior, ref) < 1e-12, (it, name)
+                    if name == "phi":
+                        assert np.array_equal(m[0, :64], interior[63])
+                        assert np.array_equal(m[65, :64], interior[0])
+                        assert np.array_equal(m[66, :64], interior[1])
+                    elif name != "rho":
+                        assert np.array_equal(m[64], interior[0])
+            o.stage("plasma_E")
+            o.stage("plasma_r")
+            o.stage("field_rho")
+            o.iter = it + 1
