@@ -383,8 +383,12 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 // original order at the front of the bucket (their write index never
 // exceeds a read index of the same or a later chunk); leavers go to the
 // leaver buffer with one atomic per wavefront.
+// waves per SIMD the register allocation of k_push must allow
+#ifndef PUSH_MIN_WAVES
+#define PUSH_MIN_WAVES 1
+#endif
 template <typename T, bool IDS, bool PE, int U>
-__global__ __launch_bounds__(CPIC_BLOCK)
+__global__ __launch_bounds__(CPIC_BLOCK, PUSH_MIN_WAVES)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
@@ -648,7 +652,9 @@ void k_gather(PushArgs a)
 }
 
 // particles per thread and chunk in k_push
+#ifndef PUSH_U
 #define PUSH_U 4
+#endif
 
 // Places leavers in their destination bucket, or back in the reserved
 // slots of their source bucket when the destination is full.  The last
