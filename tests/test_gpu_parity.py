@@ -11,7 +11,7 @@ import pytest
 
 import cpic_amd as ca
 import pyoracle as po
-from common import rel_l2, two_stream
+from common import growth_rate, growth_window, rel_l2, seeded_two_stream, two_stream
 
 pytestmark = pytest.mark.gpu
 
@@ -267,3 +267,39 @@ def test_in_plane_B_rejected():
     with pytest.raises(ca.CpicError) as ei:
         ca.Device(16, 16, (16.0, 16.0), 0.1, [-1.0], [1.0], B=(0.0, 1.0, 0.0))
     assert ei.value.code == -1
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_two_stream_linear_growth_rate(precision):
+    """Linear-phase growth rate of a seeded two-stream mode (64x8 grid, 1024
+    ppc, v0 = 1, mode 6 near the fastest-growing wavenumber, 300 steps):
+    the rate fitted from the GPU's rho history is within 1e-3 (relative) of
+    the rate fitted from the oracle's, over the same window."""
+    nx, ny, ppc, M = 64, 8, 1024, 6
+    parts = seeded_two_stream(nx, ny, ppc, 1.0, 0.01, M, 0.003, seed=5)
+    m = float(ppc)
+    o, d = make_pair(nx, ny, (float(nx), float(ny)), 0.1, [-1.0, -1.0], [m, m], parts,
+                     precision=precision, flags=0)
+    o.pre_step()
+    d.pre_step()
+    lo, lg = [], []
+    for it in range(300):
+        o.stage("field_E")
+        lo.append(np.log(np.abs(np.fft.rfft2(o.field("rho"))[0, M]) ** 2))
+        lg.append(np.log(np.abs(np.fft.rfft2(d.field("rho"))[0, M]) ** 2))
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        o.iter = it + 1
+        d.step(it)
+    w = growth_window(lo)
+    assert w[1] - w[0] >= 20, w
+    go = growth_rate(lo, 0.1, w)
+    gg = growth_rate(lg, 0.1, w)
+    k = 2 * np.pi * M / nx
+    wb = 1 / np.sqrt(2)
+    theory = np.sqrt(-(k * k + wb * wb - wb * np.sqrt(4 * k * k + wb * wb)))
+    print(f"fp{precision}: window {w}, gamma oracle {go:.6f} gpu {gg:.6f} "
+          f"(rel {abs(gg - go) / go:.2e}); cold-beam theory {theory:.4f}", flush=True)
+    assert 0.5 * theory < go < 1.2 * theory
+    assert abs(gg - go) <= 1e-3 * go

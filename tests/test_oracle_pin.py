@@ -349,3 +349,17 @@ def test_wrap_to_L_maps_to_zero():
     o.stage("plasma_r")
     p = o.particles(0)
     assert p["x"][0] == 0.0
+
+
+def test_growth_rate_fit_recovers_known_rate():
+    """The fit used for the linear-phase growth-rate comparison recovers the
+    rate of a synthetic exp(2 gamma t) energy with noise and saturation."""
+    from common import growth_rate, growth_window
+    rng = np.random.default_rng(0)
+    dt, gamma = 0.1, 0.283
+    t = np.arange(400) * dt
+    amp = np.exp(gamma * np.minimum(t, 25.0)) * (1 + 0.3 * np.cos(1.7 * t) * np.exp(-0.5 * t))
+    le = np.log(amp ** 2 + 1.0) + 1e-3 * rng.normal(size=t.size)
+    w = growth_window(le)
+    assert w[1] - w[0] > 20
+    assert abs(growth_rate(le, dt, w) - gamma) < 1e-3 * gamma * 10
