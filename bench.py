@@ -13,6 +13,9 @@ and resident in HBM before the timed region.
 
 Prints one JSON line.  `value` = particle-steps/s of the FP32 step.
 
+`--workload adjoint` times the differentiable FP64 run (256^2, 4,194,304
+particles, 200 steps, checkpoints every 20: forward and reverse sweep).
+
 `--workload ensemble` instead times a batch of 256 independent two-stream
 plasmas in one handle (grids drawn from 32^2..256^2, 16-128 particles per
 cell, v0 swept 0.05-0.2 and sigma 0.005-0.02, FP32 particles + FP64 FFT
@@ -260,6 +263,66 @@ def run_ensemble(args, ca):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args):
+    """Differentiable run: 256^2, 4,194,304 electrons (64 ppc) two-stream,
+    B = 0, 200 steps, loss = sum of field energies, gradients of every
+    initial position and velocity with checkpoints every 20 steps (FP64).
+    Times the forward and the backward (recompute + reverse sweep)
+    separately; the initial state and the gradients cross the C ABI as host
+    arrays, so their transfers are inside the timed calls."""
+    import ctypes
+    import cpic_amd as ca
+    nx, ppc, nsteps, K = 256, 64, 200, 20
+    n = nx * nx * ppc // 2
+    rng = np.random.default_rng(args.seed)
+    x = rng.uniform(0.0, nx, 2 * n)
+    y = rng.uniform(0.0, nx, 2 * n)
+    sign = np.concatenate([np.ones(n), -np.ones(n)])
+    ux = sign * 0.1 + rng.normal(0.0, 0.01, 2 * n)
+    uy = rng.normal(0.0, 0.01, 2 * n)
+    sp = np.concatenate([np.zeros(n), np.ones(n)]).astype(np.int32)
+    desc = ca.make_desc(nx, nx, (float(nx), float(nx)), 0.1, [-1.0, -1.0], [float(ppc)] * 2,
+                        precision=64)
+    h = ctypes.c_void_p()
+    ca.check(ca.lib().cpic_adj_create(ctypes.byref(desc), 2 * n,
+                                      sp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      ctypes.byref(h)))
+    loss = ctypes.c_double()
+    g = [np.empty(2 * n) for _ in range(4)]
+
+    def fwd():
+        ca.check(ca.lib().cpic_adj_forward(h, ca._dp(x), ca._dp(y), ca._dp(ux), ca._dp(uy), nsteps,
+                                           K, ctypes.byref(loss), None))
+
+    def bwd():
+        ca.check(ca.lib().cpic_adj_backward(h, 1.0, *(ca._dp(a) for a in g)))
+
+    fwd()
+    bwd()                                      # warm-up
+    t0 = time.perf_counter()
+    fwd()
+    t1 = time.perf_counter()
+    bwd()
+    t2 = time.perf_counter()
+    ca.lib().cpic_adj_destroy(h)
+    ps = 2 * n * nsteps
+    out = {
+        "metric": "particle-steps/s of the differentiable run (forward + reverse sweep)",
+        "value": ps / (t2 - t0), "unit": "particle-steps/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 1, "ms_per_step": (t2 - t0) / nsteps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded two-stream electrons, host-generated)",
+        "config": {"workload": f"differentiable two-stream {nx}^2, {2 * n} particles, {nsteps} "
+                               f"steps, loss = sum of field energies, checkpoint every {K}",
+                   "particles": 2 * n, "grid": [nx, nx], "parallelism": "single GPU"},
+        "forward_s": t1 - t0, "backward_s": t2 - t1,
+        "forward_particle_steps_per_s": ps / (t1 - t0),
+        "backward_particle_steps_per_s": ps / (t2 - t1),
+        "loss": loss.value,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +337,8 @@ def main():
                     help="steps over which FP32 and FP64 field energies are compared (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 1 GiB device-to-device copies (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=("twostream", "ensemble"), default="twostream")
+    ap.add_argument("--workload", choices=("twostream", "ensemble", "adjoint"),
+                    default="twostream")
     ap.add_argument("--plasmas", type=int, default=256, help="ensemble: number of plasmas")
     ap.add_argument("--ens-steps", type=int, default=300, help="ensemble: steps incl. warmup")
     ap.add_argument("--seed", type=int, default=2024)
@@ -291,6 +355,9 @@ def main():
     import cpic_amd as ca
     if args.workload == "ensemble":
         run_ensemble(args, ca)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
     if args.calibrate_copy:
         copy_calibration()

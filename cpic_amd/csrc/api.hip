@@ -348,6 +348,11 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 	{
 		Geo *g = &d->hgeo[p];
 		geo_fill(&descs[p], g);
+		if(!cpic_fft_blocking_ok(g))
+		{
+			dev_free(d);
+			return CPIC_EINVAL;
+		}
 		int64_t n = (int64_t) g->nx * g->ny, hx = g->nx / 2 + 1;
 		g->node0 = nodes;
 		g->ghat0 = ghat;

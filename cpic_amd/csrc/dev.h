@@ -195,6 +195,7 @@ struct cpic_dev {
 // helpers implemented in the .hip files
 int cpic_fft_init(cpic_dev *d);
 void cpic_fft_blocking(Geo *g);
+int cpic_fft_blocking_ok(const Geo *g);
 size_t cpic_push_lds(const cpic_dev *d);
 int cpic_set_iter(cpic_dev *d, int64_t iter);
 int cpic_solve(cpic_dev *d, int want_energy);
@@ -234,6 +235,11 @@ void cpic_kt_end(cpic_dev *d, int slot);
 #define FFT_LDS_BYTES (64 * 1024)
 // grid nodes per block of the E stencil
 #define EF_NODES 2048
+// threads per block of the FFT passes; a block transforms at most
+// FFT_MAX_VALUES complex values (the LDS budget above keeps every pass
+// within it)
+#define FFT_THREADS 1024
+#define FFT_MAX_VALUES 8192
 
 static inline int ilog2(int64_t n)
 {

@@ -32,75 +32,129 @@
 #include <stdlib.h>
 #include <string.h>
 
-static __device__ __forceinline__ int brev(int j, int lg)
-{
-	return lg ? (int) (__brev((unsigned) j) >> (32 - lg)) : 0;
-}
-
 static __device__ __forceinline__ double2 cmul(double2 a, double2 w)
 {
 	return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
 }
 
-// Decimation-in-time FFT of nbat transforms of length n = 2^lgn stored at
-// buf + b*stride; input in bit-reversed order, output natural.  w = exp(sign
-// 2 pi i k/len) taken from tw (size 2^lgN table).  Ends with a barrier.
-static __device__ void fft_dit(double2 *buf, int stride, int lgn, int nbat,
-		const double2 *__restrict__ tw, int lgN, double sign)
+// w = exp(sign * 2 pi i idx / N), 0 <= idx < N, from the half table
+// tw[k] = exp(+2 pi i k / N), k <= N/2 (the second half is -tw[idx - N/2]).
+static __device__ __forceinline__ double2 twiddle_at(const double2 *__restrict__ tw, int lgN,
+		int idx, double sign)
 {
-	int n = 1 << lgn;
-	int nbf = (n >> 1) * nbat;
-	for(int s = 1; s <= lgn; s++)
+	int half = 1 << (lgN - 1);
+	double2 w;
+	if(idx < half)
+		w = tw[idx];
+	else
 	{
-		int half = 1 << (s - 1);
-		int tsh = lgN - s;
-		for(int t = threadIdx.x; t < nbf; t += blockDim.x)
-		{
-			int bat = t >> (lgn - 1);
-			int bf = t & ((n >> 1) - 1);
-			int k = bf & (half - 1);
-			int i0 = bat * stride + ((bf >> (s - 1)) << s) + k;
-			int i1 = i0 + half;
-			double2 w = tw[k << tsh];
-			w.y *= sign;
-			double2 a = buf[i0];
-			double2 v = cmul(buf[i1], w);
-			buf[i0] = make_double2(a.x + v.x, a.y + v.y);
-			buf[i1] = make_double2(a.x - v.x, a.y - v.y);
-		}
-		__syncthreads();
+		w = tw[idx - half];
+		w.x = -w.x;
+		w.y = -w.y;
+	}
+	w.y *= sign;
+	return w;
+}
+
+// In-register R-point DFT (R = 2 or 4), exp(sign 2 pi i / R) roots.
+template <int R>
+static __device__ __forceinline__ void dft_small(double2 *v, double sign)
+{
+	if(R == 2)
+	{
+		double2 a = v[0], b = v[1];
+		v[0] = make_double2(a.x + b.x, a.y + b.y);
+		v[1] = make_double2(a.x - b.x, a.y - b.y);
+	}
+	else
+	{
+		double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+		double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+		double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+		double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+		// (i sign) * d13
+		double2 j13 = make_double2(-sign * d13.y, sign * d13.x);
+		v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+		v[1] = make_double2(d02.x + j13.x, d02.y + j13.y);
+		v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+		v[3] = make_double2(d02.x - j13.x, d02.y - j13.y);
 	}
 }
 
-// Decimation-in-frequency FFT: natural input, bit-reversed output.
-static __device__ void fft_dif(double2 *buf, int stride, int lgn, int nbat,
+// complex values a thread holds during one FFT stage
+#define FFT_MAXV 8
+// two blocks of FFT_THREADS per CU: at most 64 VGPRs
+#define FFT_MIN_WAVES 8
+static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
+
+// One Stockham stage of radix R over nbat transforms of length n = 2^lgn
+// (transform b at buf + b*stride), current span p = 2^lgp: butterfly i of a
+// transform reads x[i + t n/R], applies w^(t k) with w = exp(sign 2 pi i/(R p)),
+// k = i mod p, and writes the R-point DFT to y[(i - k) R + k + t p].  All
+// reads precede a barrier and all writes follow it, so one LDS buffer holds
+// both.  Needs (n/R)*nbat <= (FFT_MAXV/R) * FFT_THREADS.
+template <int R>
+static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
+		int nbat, int lgp, const double2 *__restrict__ tw, int lgN, double sign)
+{
+	constexpr int LGR = R == 4 ? 2 : 1;
+	constexpr int MAXB = FFT_MAXV / R;
+	const int lgnb = lgn - LGR, nb = 1 << lgnb, p = 1 << lgp;
+	const int total = nb * nbat;
+	const int tsh = lgN - LGR - lgp;       // table index of w^(t k) = t k 2^tsh
+	double2 v[MAXB][R];
+	int jo[MAXB];
+#pragma unroll
+	for(int m = 0; m < MAXB; m++)
+	{
+		int t_ = threadIdx.x + m * FFT_THREADS;
+		jo[m] = -1;
+		if(t_ < total)
+		{
+			int bat = t_ >> lgnb, i = t_ & (nb - 1);
+			int k = i & (p - 1);
+			const double2 *src = buf + bat * stride + i;
+#pragma unroll
+			for(int r = 0; r < R; r++)
+				v[m][r] = src[r * nb];
+#pragma unroll
+			for(int r = 1; r < R; r++)
+				if(k)
+					v[m][r] = cmul(v[m][r], twiddle_at(tw, lgN, (r * k) << tsh, sign));
+			dft_small<R>(v[m], sign);
+			jo[m] = bat * stride + ((i - k) << LGR) + k;
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for(int m = 0; m < MAXB; m++)
+		if(jo[m] >= 0)
+		{
+#pragma unroll
+			for(int r = 0; r < R; r++)
+				buf[jo[m] + r * p] = v[m][r];
+		}
+	__syncthreads();
+}
+
+// FFT of nbat transforms of length n = 2^lgn at buf + b*stride, natural
+// order in and out (Stockham autosort: radix-4 stages, one radix-2 stage
+// when lgn is odd), w = exp(sign 2 pi i/len).  Starts and ends at a barrier.
+static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
 		const double2 *__restrict__ tw, int lgN, double sign)
 {
-	int n = 1 << lgn;
-	int nbf = (n >> 1) * nbat;
-	for(int s = lgn; s >= 1; s--)
+	int lgp = 0;
+	if(lgn & 1)
 	{
-		int half = 1 << (s - 1);
-		int tsh = lgN - s;
-		for(int t = threadIdx.x; t < nbf; t += blockDim.x)
-		{
-			int bat = t >> (lgn - 1);
-			int bf = t & ((n >> 1) - 1);
-			int k = bf & (half - 1);
-			int i0 = bat * stride + ((bf >> (s - 1)) << s) + k;
-			int i1 = i0 + half;
-			double2 w = tw[k << tsh];
-			w.y *= sign;
-			double2 a = buf[i0], b = buf[i1];
-			buf[i0] = make_double2(a.x + b.x, a.y + b.y);
-			buf[i1] = cmul(make_double2(a.x - b.x, a.y - b.y), w);
-		}
-		__syncthreads();
+		stockham_stage<2>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		lgp = 1;
 	}
+	for(; lgp < lgn; lgp += 2)
+		stockham_stage<4>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 }
 
 // Pass 1: R rows per block; row y real of length nx = 2m.
-__global__ __launch_bounds__(CPIC_BLOCK)
+__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
 void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const double *__restrict__ rho_all, double2 *__restrict__ ghat_all,
 		const double2 *__restrict__ tw, int lgN)
@@ -119,12 +173,11 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	{
 		int r = i / m, j = i - r * m;
 		const double *row = rho + (int64_t) (y0 + r) * nx;
-		double2 z = make_double2(row[2 * j], row[2 * j + 1]);
-		sh[r * stride + brev(j, lgm)] = z;
+		sh[r * stride + j] = make_double2(row[2 * j], row[2 * j + 1]);
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_dit(sh, stride, lgm, R, tw, lgN, -1.0);
+		fft_block(sh, stride, lgm, R, tw, lgN, -1.0);
 
 	// X[k] = E + W^k O, X[m-k] = conj(E - W^k O), W = exp(-2 pi i/nx)
 	int npair = (m >> 1) + 1;
@@ -169,14 +222,14 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 }
 
 // Pass 2: C columns (kx) per block, each ny long and contiguous.
-__global__ __launch_bounds__(CPIC_BLOCK)
+__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
 void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		double2 *__restrict__ ghat_all, const double *__restrict__ costab_all,
 		const double2 *__restrict__ tw, int lgN, DiagSlot *__restrict__ diag,
 		int ring, const StepState *__restrict__ st)
 {
 	extern __shared__ double2 sh[];
-	__shared__ double red[CPIC_BLOCK];
+	__shared__ double red[FFT_THREADS];
 	int p = find_plasma(pre, P, blockIdx.x);
 	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, C = g.C;
@@ -192,13 +245,10 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	double escale = -0.5 * g.e0 * invN;
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-	{
-		int c = i >> lgny, j = i & (ny - 1);
-		sh[c * ny + brev(j, lgny)] = ghat[(int64_t) (k0 + c) * ny + j];
-	}
+		sh[i] = ghat[(int64_t) k0 * ny + i];
 	__syncthreads();
 	if(lgny > 0)
-		fft_dit(sh, ny, lgny, ncol, tw, lgN, -1.0);
+		fft_block(sh, ny, lgny, ncol, tw, lgN, -1.0);
 
 	double e = 0.0;
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
@@ -218,7 +268,7 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	{
 		red[threadIdx.x] = e;
 		__syncthreads();
-		for(int s = CPIC_BLOCK / 2; s > 0; s >>= 1)
+		for(int s = FFT_THREADS / 2; s > 0; s >>= 1)
 		{
 			if((int) threadIdx.x < s)
 				red[threadIdx.x] += red[threadIdx.x + s];
@@ -229,18 +279,15 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgny > 0)
-		fft_dif(sh, ny, lgny, ncol, tw, lgN, +1.0);
+		fft_block(sh, ny, lgny, ncol, tw, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-	{
-		int c = i >> lgny, j = i & (ny - 1);
-		ghat[(int64_t) (k0 + c) * ny + j] = sh[c * ny + brev(j, lgny)];
-	}
+		ghat[(int64_t) k0 * ny + i] = sh[i];
 }
 
 // Pass 3: R rows per block, c2r along X (imaginary parts of X[0] and
 // X[nx/2] ignored, as for a Hermitian input).
-__global__ __launch_bounds__(CPIC_BLOCK)
+__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
 void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const double2 *__restrict__ ghat_all, double *__restrict__ phi_all,
 		const double2 *__restrict__ tw, int lgN)
@@ -300,12 +347,12 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_dif(sh, stride, lgm, R, tw, lgN, +1.0);
+		fft_block(sh, stride, lgm, R, tw, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
 	{
 		int r = i / m, j = i - r * m;
-		double2 z = sh[r * stride + brev(j, lgm)];
+		double2 z = sh[r * stride + j];
 		double *row = phi + (int64_t) (y0 + r) * nx;
 		row[2 * j] = z.x;
 		row[2 * j + 1] = z.y;
@@ -470,6 +517,14 @@ cpic_fft_blocking(Geo *g)
 	g->C = (int) C;
 }
 
+// Whether the blocking keeps every FFT pass within FFT_MAX_VALUES per block.
+int
+cpic_fft_blocking_ok(const Geo *g)
+{
+	return (int64_t) g->R * (g->nx / 2) <= FFT_MAX_VALUES &&
+		(int64_t) g->C * g->ny <= FFT_MAX_VALUES;
+}
+
 // phi = S rho for every plasma of the batch (rho and phi laid out like the
 // state's own buffers); with diag, the field energy of each plasma goes to
 // its slot of the current step.
@@ -481,20 +536,20 @@ solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag)
 	const int *pre = d->pre;
 
 	slot = cpic_kt_begin(d, "fft_rows_r2c");
-	hipLaunchKernelGGL(k_rows_r2c, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
+	hipLaunchKernelGGL(k_rows_r2c, dim3(d->rowblocks), dim3(FFT_THREADS), d->lds_rows, d->stream,
 			d->geo, pre + PRE_ROW * (P + 1), P, rho, d->ghat, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
 	slot = cpic_kt_begin(d, "fft_cols_green");
-	hipLaunchKernelGGL(k_cols, dim3(d->colblocks), dim3(CPIC_BLOCK), d->lds_cols, d->stream,
+	hipLaunchKernelGGL(k_cols, dim3(d->colblocks), dim3(FFT_THREADS), d->lds_cols, d->stream,
 			d->geo, pre + PRE_COL * (P + 1), P, d->ghat, d->costab, d->tw, d->lgN, diag,
 			d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
 	slot = cpic_kt_begin(d, "fft_rows_c2r");
-	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(CPIC_BLOCK), d->lds_rows, d->stream,
+	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(FFT_THREADS), d->lds_rows, d->stream,
 			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, phi, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());

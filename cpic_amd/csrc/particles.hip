@@ -383,12 +383,36 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 // original order at the front of the bucket (their write index never
 // exceeds a read index of the same or a later chunk); leavers go to the
 // leaver buffer with one atomic per wavefront.
-// waves per SIMD the register allocation of k_push must allow
-#ifndef PUSH_MIN_WAVES
-#define PUSH_MIN_WAVES 1
+// Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
+// thread tid takes base + u*CPIC_BLOCK + tid (coalesced SoA loads).
+template <typename T, bool IDS, int U>
+static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
+		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
+{
+#pragma unroll
+	for(int u = 0; u < U; u++)
+	{
+		int j = base + u * CPIC_BLOCK + tid;
+		act[u] = j < ntot;
+		int64_t slot = bstart + j + (j >= n1 ? seg2 : 0);
+		if(act[u])
+			load_p<T>(p, slot, q[u], IDS);
+	}
+}
+
+// 1: k_push issues the next chunk's loads before pushing the current one
+#ifndef PUSH_PREFETCH
+#define PUSH_PREFETCH 0
+#endif
+
+// Waves per SIMD the register allocation of k_push must allow.  Measured at
+// 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (79 VGPRs with
+// U = 2), the FP64 kernel loses from any bound (its state is twice the size).
+#ifndef PUSH_MIN_WAVES_F32
+#define PUSH_MIN_WAVES_F32 6
 #endif
 template <typename T, bool IDS, bool PE, int U>
-__global__ __launch_bounds__(CPIC_BLOCK, PUSH_MIN_WAVES)
+__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
@@ -438,20 +462,30 @@ void k_push(PushArgs a)
 		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 		int w = 0;
 
+#if PUSH_PREFETCH
+		PState<T> nq[U];
+		bool nact[U];
+		load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, 0, tid, nq, nact);
+#endif
 		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
 		{
 			PState<T> q[U];
 			bool act[U], leave[U];
 			int ndst[U];
+#if PUSH_PREFETCH
+			// the next chunk's loads are in flight while this one is pushed;
+			// its slots lie beyond everything this chunk writes
 #pragma unroll
 			for(int u = 0; u < U; u++)
 			{
-				int j = base + u * CPIC_BLOCK + tid;
-				act[u] = j < ntot;
-				int64_t slot = bstart + j + (j >= n1 ? seg2 : 0);
-				if(act[u])
-					load_p<T>(a.p, slot, q[u], IDS);
+				q[u] = nq[u];
+				act[u] = nact[u];
 			}
+			load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base + U * CPIC_BLOCK, tid, nq,
+					nact);
+#else
+			load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
+#endif
 #pragma unroll
 			for(int u = 0; u < U; u++)
 			{
@@ -651,9 +685,10 @@ void k_gather(PushArgs a)
 	}
 }
 
-// particles per thread and chunk in k_push
+// particles per thread and chunk in k_push (2 and 4 measure within 1-2 %;
+// 2 leaves room for the FP32 occupancy bound above)
 #ifndef PUSH_U
-#define PUSH_U 4
+#define PUSH_U 2
 #endif
 
 // Places leavers in their destination bucket, or back in the reserved
