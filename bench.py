@@ -98,22 +98,20 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(w, steps=10):
-    """The FP64 oracle (C restatement of the reference loop, OpenMP over 64
-    chunks) on the full grid with 1/64 of the particles; the step time of the
-    full workload is extrapolated as t_fields + 64 * t_particles."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-    frac = 64
+def oracle_sample(po, w, frac, steps, seed=7):
+    """Times the oracle's steps on the full grid with 1/frac of the
+    particles: (field seconds per step, particle seconds per step, sample
+    size)."""
     grid = w["grid"]
     o = po.Oracle(grid, grid, (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]), nchunks=64)
     nsample = 0
     for s in range(2):
         n = w["n"][s] // frac
-        rng = np.random.default_rng([7, s])
+        rng = np.random.default_rng([seed, s])
         x, y, ux, uy = gen_species(rng, n, w["L"], w["v0"][s], w["sigma"][s])
         o.add(s, np.arange(n, dtype=np.int64), x, y, ux, uy)
         nsample += n
+        del x, y, ux, uy
     o.pre_step()
     o.step()                                  # iteration 0
     tf = tp = 0.0
@@ -128,21 +126,46 @@ def cpu_baseline(w, steps=10):
         o.iter = o.iter + 1
         tf += t1 - t0
         tp += t2 - t1
-    tf /= steps
-    tp /= steps
+    del o
+    return tf / steps, tp / steps, nsample
+
+
+def cpu_baseline(w, frac=8, steps=10):
+    """The FP64 oracle (C restatement of the reference loop, OpenMP over 64
+    chunks) on the host cores: the full grid with 1/frac of the particles;
+    the full workload's step time is extrapolated as t_fields +
+    frac * t_particles.  Also one core on a 1/256 sample."""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tf, tp, ns = oracle_sample(po, w, frac, steps)
     t_full = tf + frac * tp
     ntot = sum(w["n"])
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
+    single = None
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        tf1, tp1, ns1 = oracle_sample(po, w, 256, 2)
+        gomp.omp_set_num_threads(threads)
+        single = {"value": ntot / (tf1 + 256 * tp1), "cores": 1,
+                  "ns_per_particle_step": tp1 / ns1 * 1e9, "fields_ms": tf1 * 1e3}
+    except OSError:
+        pass
+    out = {
         "value": ntot / t_full,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP64 on {grid}^2 with {nsample} particles (1/{frac} of the workload), "
+        "sample": (f"oracle FP64 on {w['grid']}^2 with {ns} particles (1/{frac} of the workload), "
                    f"{steps} steps: fields {tf * 1e3:.1f} ms/step, particles {tp * 1e3:.1f} "
-                   f"ms/step ({tp / nsample * 1e9:.1f} ns/particle-step); full step extrapolated "
-                   f"to {t_full:.2f} s; CPU: {cpu_model()}, {threads} threads, radix-2 FFT"),
+                   f"ms/step ({tp / ns * 1e9:.2f} ns/particle-step); full step extrapolated "
+                   f"to {t_full:.2f} s; CPU: {cpu_model()}, {threads} threads (nproc "
+                   f"{os.cpu_count()}), radix-2 FFT"),
     }
+    if single:
+        out["single_core"] = single
+    return out
 
 
 def traffic_from_profiles(precision):

@@ -154,6 +154,7 @@ dev_free(cpic_dev *d)
 	cpic_parts_free(d, 1);
 	hipFree(d->bk.start); hipFree(d->bk.cap); hipFree(d->bk.n1); hipFree(d->bk.res);
 	hipFree(d->bk.n2);
+	hipFree(d->order);
 	hipFree(d->lv.x); hipFree(d->lv.y); hipFree(d->lv.ux); hipFree(d->lv.uy);
 	hipFree(d->lv.id); hipFree(d->lv.Ex); hipFree(d->lv.Ey); hipFree(d->lv.dst);
 	hipFree(d->lv.src); hipFree(d->lv.count);
@@ -421,6 +422,7 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 			(ret = alloc_zero((void **) &d->st, sizeof(StepState))) ||
 			(ret = alloc_zero((void **) &d->geo, sizeof(Geo) * P)) ||
 			(ret = alloc_zero((void **) &d->pre, sizeof(int) * PRE_COUNT * (P + 1))) ||
+			(ret = alloc_zero((void **) &d->order, sizeof(int) * d->ntiles)) ||
 			(ret = alloc_zero((void **) &d->bk.start, sizeof(int64_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.cap, sizeof(int32_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.n1, sizeof(int32_t) * d->nb)) ||
@@ -441,6 +443,11 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 		return CPIC_EHIP;
 	}
 	(void) k;
+	if((ret = cpic_parts_order(d, NULL)))
+	{
+		dev_free(d);
+		return ret;
+	}
 	d->rho_cur = 0;
 	d->rho_next_ready = 1;   // both buffers start zeroed
 	d->last_iter = -1;

@@ -157,6 +157,7 @@ struct cpic_dev {
 	int pcur;
 	int64_t pcap[2];
 	Buckets bk;
+	int *order;              // [ntiles] tile taken by each block of the tile kernels
 	Leavers lv;
 	int step_parity;
 	int64_t *count;          // [P * CPIC_MAX_SPECIES]
@@ -203,6 +204,7 @@ int cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi);
 int cpic_efield_ptr(cpic_dev *d, const double *phi, void *E);
 int cpic_efield(cpic_dev *d);
 int cpic_parts_init(cpic_dev *d);
+int cpic_parts_order(cpic_dev *d, const int *cnt);
 int cpic_parts_rebuild(cpic_dev *d);
 int cpic_parts_push(cpic_dev *d, int first_iter);
 int cpic_parts_deposit(cpic_dev *d, double *rho_out);
@@ -231,8 +233,9 @@ void cpic_kt_end(cpic_dev *d, int slot);
 // Dynamic LDS limit requested for kernels with large tiles or rows: the
 // 160 KB of a gfx950 workgroup minus room for static LDS.
 #define CPIC_LDS_LIMIT (160 * 1024 - 8 * 1024)
-// LDS budget used to choose FFT rows/columns per block
+// LDS budgets used to choose FFT columns and rows per block
 #define FFT_LDS_BYTES (64 * 1024)
+#define FFT_ROW_LDS_BYTES (76 * 1024)
 // grid nodes per block of the E stencil
 #define EF_NODES 2048
 // threads per block of the FFT passes; a block transforms at most

@@ -498,10 +498,14 @@ cpic_fft_init(cpic_dev *d)
 void
 cpic_fft_blocking(Geo *g)
 {
+	// rows: up to FFT_ROW_LDS_BYTES of LDS and FFT_THREADS*4 values (one
+	// radix-4 butterfly per thread), so two blocks share a CU
 	int64_t per = (int64_t) (g->nx / 2 + 1) * (int64_t) sizeof(double2);
-	int64_t R = FFT_LDS_BYTES / per, r = 1;
-	if(R > 8)
-		R = 8;
+	int64_t R = FFT_ROW_LDS_BYTES / per, r = 1;
+	if(R * (g->nx / 2) > FFT_THREADS * 4)
+		R = FFT_THREADS * 4 / (g->nx / 2 > 0 ? g->nx / 2 : 1);
+	if(R < 1)
+		R = 1;
 	if(R > g->ny)
 		R = g->ny;
 	while(r * 2 <= R)
@@ -517,11 +521,14 @@ cpic_fft_blocking(Geo *g)
 	g->C = (int) C;
 }
 
-// Whether the blocking keeps every FFT pass within FFT_MAX_VALUES per block.
+// Whether the blocking keeps every FFT pass within FFT_MAX_VALUES values and
+// the LDS limit per block.
 int
 cpic_fft_blocking_ok(const Geo *g)
 {
-	return (int64_t) g->R * (g->nx / 2) <= FFT_MAX_VALUES &&
+	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= CPIC_LDS_LIMIT &&
+		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= CPIC_LDS_LIMIT &&
+		(int64_t) g->R * (g->nx / 2) <= FFT_MAX_VALUES &&
 		(int64_t) g->C * g->ny <= FFT_MAX_VALUES;
 }
 

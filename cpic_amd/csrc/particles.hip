@@ -31,6 +31,7 @@
 #include "dev.h"
 #include "picmath.h"
 
+#include <algorithm>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -39,12 +40,13 @@ template <typename T> struct V2;
 template <> struct V2<double> { typedef double2 t; };
 template <> struct V2<float> { typedef float2 t; };
 
-// Arguments of the tile kernels.  Every block handles one tile of the batch
-// (tile_map of the block index), finds the tile's plasma in pre_tile and
-// takes the plasma's geometry and constants from geo.
+// Arguments of the tile kernels.  Block b handles batch tile order[b], finds
+// the tile's plasma in pre_tile and takes the plasma's geometry and
+// constants from geo.
 struct PushArgs {
 	const Geo *geo;
 	const int *pre_tile;  // [P+1] first global tile of each plasma
+	const int *order;     // [ntiles] tile of each block (cpic_parts_order)
 	int P;
 	int ntiles;           // tiles of the whole batch
 	Buckets bk;
@@ -93,10 +95,11 @@ static __device__ __forceinline__ int wrapi(int i, int n)
 	return i >= n ? i - n : i;
 }
 
-static __device__ __forceinline__ int tile_map(int b, int ntiles)
+// Block -> tile map for tiles of similar size: contiguous tile bands go to
+// each of the 8 XCDs (workgroups are dispatched round-robin), so
+// neighbouring tiles share an L2.
+static inline int tile_map(int b, int ntiles)
 {
-	// spread contiguous tile bands over the 8 XCDs (workgroups are
-	// dispatched round-robin), so neighbouring tiles share an L2
 	if((ntiles & 7) == 0)
 		return (b & 7) * (ntiles >> 3) + (b >> 3);
 	return b;
@@ -115,6 +118,11 @@ static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
 	int64_t s = bk.start[b];
 	return j < n1 ? s + j : s + (bk.cap[b] - bk.res[b]) + (j - n1);
 }
+
+// 1: particle-state stores bypass the caches' retention (nontemporal)
+#ifndef PUSH_NT_STORE
+#define PUSH_NT_STORE 0
+#endif
 
 // Per-particle state in registers.
 template <typename T> struct PState {
@@ -139,10 +147,18 @@ template <typename T>
 static __device__ __forceinline__ void store_p(const Parts &p, int64_t s, const PState<T> &q,
 		bool ids, bool pe)
 {
+#if PUSH_NT_STORE
+	// streaming stores: the state is read back only in the next step
+	__builtin_nontemporal_store(q.x, &((T *) p.x)[s]);
+	__builtin_nontemporal_store(q.y, &((T *) p.y)[s]);
+	__builtin_nontemporal_store(q.ux, &((T *) p.ux)[s]);
+	__builtin_nontemporal_store(q.uy, &((T *) p.uy)[s]);
+#else
 	((T *) p.x)[s] = q.x;
 	((T *) p.y)[s] = q.y;
 	((T *) p.ux)[s] = q.ux;
 	((T *) p.uy)[s] = q.uy;
+#endif
 	if(ids)
 		p.id[s] = q.id;
 	if(pe)
@@ -421,7 +437,7 @@ void k_push(PushArgs a)
 	__shared__ int s_kcount;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
 
-	int gt = tile_map(blockIdx.x, a.ntiles);
+	int gt = a.order[blockIdx.x];
 	int pl = find_plasma(a.pre_tile, a.P, gt);
 	// the plasma's table entry, read once into registers (a reference into
 	// global memory would be re-read after every particle store); the
@@ -625,7 +641,7 @@ __global__ __launch_bounds__(CPIC_BLOCK)
 void k_deposit(PushArgs a)
 {
 	extern __shared__ unsigned char smem[];
-	int gt = tile_map(blockIdx.x, a.ntiles);
+	int gt = a.order[blockIdx.x];
 	const Geo *gp = &a.geo[find_plasma(a.pre_tile, a.P, gt)];
 	const Geo g = *gp;
 	double *sR = (double *) smem;
@@ -660,7 +676,7 @@ void k_gather(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
 	extern __shared__ unsigned char smem[];
-	int gt = tile_map(blockIdx.x, a.ntiles);
+	int gt = a.order[blockIdx.x];
 	const Geo g = a.geo[find_plasma(a.pre_tile, a.P, gt)];
 	T2 *sE = (T2 *) smem;
 	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
@@ -1134,6 +1150,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	memset(&a, 0, sizeof(a));
 	a.geo = d->geo;
 	a.pre_tile = PRE(d, PRE_TILE);
+	a.order = d->order;
 	a.P = d->P;
 	a.ntiles = d->ntiles;
 	a.bk = d->bk;
@@ -1232,6 +1249,46 @@ cpic_parts_gather(cpic_dev *d)
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
+}
+
+// Order in which the tile kernels' blocks take the batch's tiles.  With
+// tiles of similar particle counts (max/min <= 1.5): the XCD-banded map
+// above.  Otherwise largest first (longest-processing-time scheduling:
+// the big tiles of a batch start early instead of forming the tail of the
+// launch).  cnt[b] = particles of bucket b (host), or NULL for the banded map.
+int
+cpic_parts_order(cpic_dev *d, const int *cnt)
+{
+	const int nt = d->ntiles;
+	int *ord = (int *) malloc(sizeof(int) * (size_t) nt);
+	int64_t *tc = NULL;
+	if(!ord)
+		return CPIC_ENOMEM;
+	for(int b = 0; b < nt; b++)
+		ord[b] = tile_map(b, nt);
+	if(cnt && (tc = (int64_t *) calloc((size_t) nt, sizeof(int64_t))))
+	{
+		int64_t lo = INT64_MAX, hi = 0;
+		for(int p = 0; p < d->P; p++)
+		{
+			const Geo &g = d->hgeo[p];
+			for(int t = 0; t < g.ntiles; t++)
+			{
+				int64_t c = 0;
+				for(int s = 0; s < g.ns; s++)
+					c += cnt[g.bucket0 + t * g.ns + s];
+				tc[g.tile0 + t] = c;
+				lo = c < lo ? c : lo;
+				hi = c > hi ? c : hi;
+			}
+		}
+		if(hi > lo + lo / 2)
+			std::stable_sort(ord, ord + nt, [tc](int a, int b) { return tc[a] > tc[b]; });
+		free(tc);
+	}
+	hipError_t e = hipMemcpy(d->order, ord, sizeof(int) * (size_t) nt, hipMemcpyHostToDevice);
+	free(ord);
+	return e == hipSuccess ? 0 : CPIC_EHIP;
 }
 
 // Full re-sort of the batch: counts per destination bucket (current layout
@@ -1368,6 +1425,20 @@ cpic_parts_rebuild(cpic_dev *d)
 		HIPCHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
 		HIPCHK(hipStreamSynchronize(d->stream));
 		d->pcur = spare;
+		// block order for the new counts
+		int *hc = (int *) malloc(sizeof(int) * (size_t) nb);
+		if(!hc)
+			goto out;
+		if(hipMemcpy(hc, d->bk.n1, sizeof(int) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+		{
+			free(hc);
+			ret = CPIC_EHIP;
+			goto out;
+		}
+		ret = cpic_parts_order(d, hc);
+		free(hc);
+		if(ret)
+			goto out;
 	}
 	for(k = 0; k < P * MS; k++)
 	{

@@ -1302,12 +1302,28 @@ orc_set_field(orc_sim_t *s, int which, const double *in)
 double
 orc_field_energy(const orc_sim_t *s)
 {
+	/* -(e0/2) sum rho*phi.  The products have both signs and cancel, so the
+	 * sum is accumulated as if in twice the working precision (Ogita, Rump
+	 * and Oishi's Dot2: exact products by fma, error-free sums), which
+	 * makes the oracle's value accurate to the last bits whatever the grid
+	 * size. */
 	int64_t nx = s->nx, ny = s->ny, ix, iy;
-	double acc = 0.0;
+	double sum = 0.0, err = 0.0;
 	for(iy = 0; iy < ny; iy++)
+	{
+		const double *r = &s->rho[iy * s->rs], *f = phi_row(s, iy);
 		for(ix = 0; ix < nx; ix++)
-			acc += s->rho[iy * s->rs + ix] * phi_row(s, iy)[ix];
-	return -0.5 * s->p.e0 * acc;
+		{
+			double p = r[ix] * f[ix];
+			double pe = fma(r[ix], f[ix], -p);
+			double t = sum + p;
+			double z = t - sum;
+			double te = (sum - (t - z)) + (p - z);
+			sum = t;
+			err += te + pe;
+		}
+	}
+	return -0.5 * s->p.e0 * (sum + err);
 }
 
 double

@@ -303,3 +303,41 @@ def test_two_stream_linear_growth_rate(precision):
           f"(rel {abs(gg - go) / go:.2e}); cold-beam theory {theory:.4f}", flush=True)
     assert 0.5 * theory < go < 1.2 * theory
     assert abs(gg - go) <= 1e-3 * go
+
+
+def test_c2_scale_fp64_matches_oracle():
+    """The FP64 drop-in path at scale: 512^2, 2 x 8,388,608 counter-streaming
+    electrons (64 per cell), Bz with w_c/w_p = 0.5, dt = 0.05; 20 steps
+    against the oracle at 1e-12 (rho, phi, E, particles, field energy), then
+    on to 500 steps on the GPU with charge and particle-count invariants."""
+    import time
+    nx = 512
+    n = nx * nx * 64 // 2
+    L = (float(nx), float(nx))
+    m = 64.0
+    t0 = time.time()
+    parts = two_stream(n, L, 0.1, 0.01, seed=77)
+    o, d = make_pair(nx, nx, L, 0.05, [-1.0, -1.0], [m, m], parts, B=(0.0, 0.0, 32.0),
+                     nchunks=8, flags=ca.F_IDS)
+    del parts
+    print(f"setup {time.time() - t0:.1f} s", flush=True)
+    W = run_both(o, d, 20)
+    d.sync()
+    print(f"20 steps on both {time.time() - t0:.1f} s", flush=True)
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert rel_l2(d.field(f), o.field(f)) < 1e-12, f
+    for s in range(2):
+        a, b = o.particles(s), d.particles(s)
+        assert np.array_equal(a["id"], b["id"])
+        for k in ("x", "y", "ux", "uy"):
+            assert rel_l2(b[k], a[k]) < 1e-12, (s, k)
+    del o
+    e = d.energy()
+    assert rel_l2(e["field"], W) < 1e-12
+    d.run(20, 480)
+    d.sync()
+    rho = d.field("rho")
+    assert abs(rho.sum() - 2 * n) < 1e-9 * 2 * n       # -q/e0 = 1 per electron
+    assert d.count(0) == n and d.count(1) == n
+    assert np.all(np.isfinite(d.energy()["field"][:500]))
+    print(f"500 steps {time.time() - t0:.1f} s", flush=True)
