@@ -803,7 +803,8 @@ static int
 graph_steps(cpic_dev *d, int64_t iter)
 {
 	int ret;
-	if(d->graph_ok && (d->graph_parity != d->step_parity || d->graph_rho != d->rho_cur))
+	if(d->graph_ok && (d->graph_parity != d->step_parity || d->graph_rho != d->rho_cur ||
+				d->graph_pcur != d->pcur))
 		return 1;   // out of phase: caller runs one step eagerly
 	if(!d->graph_ok)
 	{
@@ -811,6 +812,7 @@ graph_steps(cpic_dev *d, int64_t iter)
 		graph_drop(d);
 		d->graph_parity = d->step_parity;
 		d->graph_rho = d->rho_cur;
+		d->graph_pcur = d->pcur;
 		HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
 		d->capturing = 1;
 		ret = cpic_dev_step(d, iter);

@@ -182,6 +182,7 @@ struct cpic_dev {
 	hipGraphExec_t graph;
 	int graph_parity;        // step_parity at capture
 	int graph_rho;           // rho_cur at capture
+	int graph_pcur;          // pcur at capture
 	int graph_ok;
 	int capturing;           // inside hipStreamBeginCapture/EndCapture
 

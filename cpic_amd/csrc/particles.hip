@@ -50,7 +50,8 @@ struct PushArgs {
 	int P;
 	int ntiles;           // tiles of the whole batch
 	Buckets bk;
-	Parts p;
+	Parts p;              // particles read (and, in place, written)
+	Parts q;              // particles written when PUSH_PINGPONG
 	Leavers lv;
 	int par;              // leaver counter parity
 	int first;            // iteration 0: dt' = -dt/2 and no position update
@@ -421,6 +422,13 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #define PUSH_PREFETCH 0
 #endif
 
+// 1: k_push reads one particle buffer and writes the other (the two swap
+// every step); each wavefront then claims its stayers' slots with one LDS
+// atomic instead of a block-wide scan and barrier per chunk
+#ifndef PUSH_PINGPONG
+#define PUSH_PINGPONG 0
+#endif
+
 // Waves per SIMD the register allocation of k_push must allow.  Measured at
 // 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (79 VGPRs with
 // U = 2), the FP64 kernel loses from any bound (its state is twice the size).
@@ -435,6 +443,7 @@ void k_push(PushArgs a)
 	extern __shared__ unsigned char smem[];
 	__shared__ int s_cnt[2][U][CPIC_BLOCK / 64];
 	__shared__ int s_kcount;
+	__shared__ int s_w;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
 
 	int gt = a.order[blockIdx.x];
@@ -460,7 +469,10 @@ void k_push(PushArgs a)
 
 	tile_setup<T>(g, ox, oy, E, sE, sR);
 	if(tid == 0)
+	{
 		s_kcount = 0;
+		s_w = 0;
+	}
 	__syncthreads();
 
 	double ke = 0.0, mpx = 0.0, mpy = 0.0;
@@ -553,6 +565,24 @@ void k_push(PushArgs a)
 						atomicAdd(&s_kcount, __popcll(placed));
 				}
 			}
+#if PUSH_PINGPONG
+			// stayers: one LDS atomic per wavefront claims their slots in
+			// the other buffer (order within the bucket is not kept)
+#pragma unroll
+			for(int u = 0; u < U; u++)
+			{
+				unsigned long long smk = __ballot(act[u] && !leave[u]);
+				if(smk)
+				{
+					int wb = 0;
+					if(lane == 0)
+						wb = atomicAdd(&s_w, __popcll(smk));
+					wb = __shfl(wb, 0);
+					if(act[u] && !leave[u])
+						store_p<T>(a.q, bstart + wb + __popcll(smk & lt), q[u], IDS, PE);
+				}
+			}
+#else
 			// stayers: block-wide stable compaction
 			unsigned long long sm[U];
 #pragma unroll
@@ -581,12 +611,18 @@ void k_push(PushArgs a)
 			}
 			w = run;
 			buf ^= 1;
+#endif
 		}
 
 		__syncthreads();
 		if(tid == 0)
 		{
+#if PUSH_PINGPONG
+			a.bk.n1[b] = s_w;
+			s_w = 0;
+#else
 			a.bk.n1[b] = w;
+#endif
 			a.bk.res[b] = s_kcount;
 			a.bk.n2[b] = 0;
 			s_kcount = 0;
@@ -1155,6 +1191,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	a.ntiles = d->ntiles;
 	a.bk = d->bk;
 	a.p = d->p[d->pcur];
+	a.q = d->p[1 - d->pcur];
 	a.lv = d->lv;
 	a.par = d->step_parity;
 	a.first = first;
@@ -1210,6 +1247,9 @@ cpic_parts_push(cpic_dev *d, int first_iter)
 		DISPATCH_U(k_push, float, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
+#if PUSH_PINGPONG
+	d->pcur = 1 - d->pcur;
+#endif
 
 	slot = cpic_kt_begin(d, "append_leavers");
 	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->geo, PRE(d, PRE_BUCKET),
@@ -1425,6 +1465,12 @@ cpic_parts_rebuild(cpic_dev *d)
 		HIPCHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
 		HIPCHK(hipStreamSynchronize(d->stream));
 		d->pcur = spare;
+#if PUSH_PINGPONG
+		// the push writes the other buffer: it needs the same capacity
+		if(d->pcap[1 - spare] < d->pcap[spare])
+			if((ret = cpic_parts_alloc(d, 1 - spare, d->pcap[spare])))
+				goto out;
+#endif
 		// block order for the new counts
 		int *hc = (int *) malloc(sizeof(int) * (size_t) nb);
 		if(!hc)
