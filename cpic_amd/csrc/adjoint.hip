@@ -44,7 +44,10 @@ struct cpic_adj {
 	int64_t n;
 	int ns;
 	AdjSpecies k[2];          // [0]: iteration 0 (dt' = -dt/2), [1]: later iterations
-	int32_t *sp;              // species of each particle
+	int32_t *sp;              // species of each particle (device, in sorted order)
+	int32_t *hsp;             // species of each particle (host, caller's order)
+	int64_t *perm;            // sorted position i holds the caller's particle perm[i]
+	double *hbuf;             // host staging, 4n doubles
 	double *x, *y, *ux, *uy;  // state
 	double *rho[2], *phi, *psi, *lrho;
 	double2 *E, *lE;
@@ -75,24 +78,157 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 	i11 = i1y * g.nx + i1x;
 }
 
-// rho += c * w at the 4 nodes (order of src/interpolate.c:259-262)
+// Deposits.  The particles are kept sorted by their initial cell, so the
+// ADJ_PPT*CPIC_BLOCK consecutive particles of a block lie in a small patch
+// of cells: each block accumulates its deposit in LDS over the bounding box
+// of its cells and flushes the non-zero nodes with one global atomic each.
+// A block whose box is too large or wraps around the grid deposits with
+// global atomics directly.
+#define ADJ_PPT 8
+#define ADJ_CHUNK (ADJ_PPT * CPIC_BLOCK)
+#define ADJ_BOX_NODES 3072          // 48 KB of double2, 24 KB of double
+
+struct Box {
+	int x0, y0, w, h;
+	bool lds;
+};
+
+// Bounding box (plus the CIC +1 node) of the block's cells [lo, hi].
+static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy, int *s4)
+{
+	if(threadIdx.x == 0)
+	{
+		s4[0] = 0x7fffffff;
+		s4[1] = -1;
+		s4[2] = 0x7fffffff;
+		s4[3] = -1;
+	}
+	__syncthreads();
+	if(hix >= 0)
+	{
+		atomicMin(&s4[0], lox);
+		atomicMax(&s4[1], hix);
+		atomicMin(&s4[2], loy);
+		atomicMax(&s4[3], hiy);
+	}
+	__syncthreads();
+	Box b;
+	b.x0 = s4[0];
+	b.y0 = s4[2];
+	b.w = s4[1] - s4[0] + 2;
+	b.h = s4[3] - s4[2] + 2;
+	b.lds = s4[1] >= 0 && s4[1] - s4[0] <= g.nx / 2 && s4[3] - s4[2] <= g.ny / 2 &&
+		b.w * b.h <= ADJ_BOX_NODES;
+	__syncthreads();
+	return b;
+}
+
+static __device__ __forceinline__ void cell_of(const Geo &g, double x, double y, int &ix, int &iy,
+		double &rx, double &ry)
+{
+	cell64(g, x, y, ix, iy, rx, ry);
+}
+
+// node of the box (dx, dy in {0,1} from cell (ix, iy)), or the global node
+static __device__ __forceinline__ int box_node(const Box &b, int ix, int iy, int dx, int dy)
+{
+	return (iy - b.y0 + dy) * b.w + (ix - b.x0 + dx);
+}
+
+static __device__ __forceinline__ int64_t wrap_node(const Geo &g, int x, int y)
+{
+	x = x >= g.nx ? x - g.nx : x;
+	y = y >= g.ny ? y - g.ny : y;
+	return (int64_t) y * g.nx + x;
+}
+
+// Adds c * (CIC weights of (rx, ry)) at the 4 nodes of cell (ix, iy) into
+// the LDS box or, without one, into the global grid (order of
+// src/interpolate.c:259-262).
+static __device__ __forceinline__ void dep1(const Geo &g, const Box &b, double *sacc,
+		double *__restrict__ rho, int ix, int iy, double rx, double ry, double c)
+{
+	double dxw = 1.0 - rx, dyw = 1.0 - ry;
+	double v00 = (dxw * dyw) * c, v01 = (dxw * ry) * c, v10 = (rx * dyw) * c, v11 = (rx * ry) * c;
+	if(b.lds)
+	{
+		atomicAdd(&sacc[box_node(b, ix, iy, 0, 0)], v00);
+		atomicAdd(&sacc[box_node(b, ix, iy, 0, 1)], v01);
+		atomicAdd(&sacc[box_node(b, ix, iy, 1, 0)], v10);
+		atomicAdd(&sacc[box_node(b, ix, iy, 1, 1)], v11);
+	}
+	else
+	{
+		unsafeAtomicAdd(&rho[wrap_node(g, ix, iy)], v00);
+		unsafeAtomicAdd(&rho[wrap_node(g, ix, iy + 1)], v01);
+		unsafeAtomicAdd(&rho[wrap_node(g, ix + 1, iy)], v10);
+		unsafeAtomicAdd(&rho[wrap_node(g, ix + 1, iy + 1)], v11);
+	}
+}
+
+static __device__ void box_zero(const Box &b, double *sacc, int ncomp)
+{
+	if(b.lds)
+		for(int i = threadIdx.x; i < b.w * b.h * ncomp; i += blockDim.x)
+			sacc[i] = 0.0;
+	__syncthreads();
+}
+
+// flushes component k of an LDS box holding ncomp interleaved components
+static __device__ void box_flush(const Geo &g, const Box &b, const double *sacc, int ncomp,
+		double *__restrict__ out, int out_stride)
+{
+	__syncthreads();
+	if(!b.lds)
+		return;
+	for(int i = threadIdx.x; i < b.w * b.h; i += blockDim.x)
+	{
+		int ly = i / b.w, lx = i - ly * b.w;
+		int64_t gnode = wrap_node(g, b.x0 + lx, b.y0 + ly);
+		for(int k = 0; k < ncomp; k++)
+		{
+			double v = sacc[i * ncomp + k];
+			if(v != 0.0)
+				unsafeAtomicAdd(&out[gnode * out_stride + k], v);
+		}
+	}
+}
+
+// rho += Dep(r) for the particles of the block
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ sp,
 		const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ rho)
 {
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-			i += (int64_t) gridDim.x * blockDim.x)
+	__shared__ double sacc[ADJ_BOX_NODES];
+	__shared__ int s4[4];
+	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
+	int ix[ADJ_PPT], iy[ADJ_PPT];
+	double rx[ADJ_PPT], ry[ADJ_PPT];
+	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
 	{
-		int i00, i01, i10, i11;
-		double rx, ry;
-		cic_nodes(g, x[i], y[i], i00, i01, i10, i11, rx, ry);
-		double c = k.c[sp[i]];
-		double dxw = 1.0 - rx, dyw = 1.0 - ry;
-		unsafeAtomicAdd(&rho[i00], (dxw * dyw) * c);
-		unsafeAtomicAdd(&rho[i01], (dxw * ry) * c);
-		unsafeAtomicAdd(&rho[i10], (rx * dyw) * c);
-		unsafeAtomicAdd(&rho[i11], (rx * ry) * c);
+		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
+		ix[u] = -1;
+		if(i < n)
+		{
+			cell_of(g, x[i], y[i], ix[u], iy[u], rx[u], ry[u]);
+			lox = min(lox, ix[u]);
+			hix = max(hix, ix[u]);
+			loy = min(loy, iy[u]);
+			hiy = max(hiy, iy[u]);
+		}
 	}
+	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	box_zero(b, sacc, 1);
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
+	{
+		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
+		if(ix[u] >= 0)
+			dep1(g, b, sacc, rho, ix[u], iy[u], rx[u], ry[u], k.c[sp[i]]);
+	}
+	box_flush(g, b, sacc, 1, rho, 1);
 }
 
 // gather + Boris + move + wrap, then deposit at the new position into rho_next
@@ -102,18 +238,31 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double *__restrict__ ux, double *__restrict__ uy, const double2 *__restrict__ E,
 		double *__restrict__ rho_next, int *derr)
 {
+	__shared__ double sacc[ADJ_BOX_NODES];
+	__shared__ int s4[4];
 	bool bad = false, nan_ = false;
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-			i += (int64_t) gridDim.x * blockDim.x)
+	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
+	int ix[ADJ_PPT], iy[ADJ_PPT];
+	double rx[ADJ_PPT], ry[ADJ_PPT];
+	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
 	{
+		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
+		ix[u] = -1;
+		if(i >= n)
+			continue;
 		int s = sp[i];
-		int i00, i01, i10, i11;
-		double rx, ry;
+		int jx, jy;
+		double qx, qy;
 		double px = x[i], py = y[i];
-		cic_nodes(g, px, py, i00, i01, i10, i11, rx, ry);
-		double dxw = 1.0 - rx, dyw = 1.0 - ry;
-		double w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
-		double2 e00 = E[i00], e01 = E[i01], e10 = E[i10], e11 = E[i11];
+		cell_of(g, px, py, jx, jy, qx, qy);
+		int j1x = jx + 1 == g.nx ? 0 : jx + 1;
+		int j1y = jy + 1 == g.ny ? 0 : jy + 1;
+		double dxw = 1.0 - qx, dyw = 1.0 - qy;
+		double w00 = dxw * dyw, w01 = dxw * qy, w10 = qx * dyw, w11 = qx * qy;
+		double2 e00 = E[(int64_t) jy * g.nx + jx], e01 = E[(int64_t) j1y * g.nx + jx];
+		double2 e10 = E[(int64_t) jy * g.nx + j1x], e11 = E[(int64_t) j1y * g.nx + j1x];
 		double ex = w00 * e00.x;
 		ex = fma(w01, e01.x, ex);
 		ex = fma(w10, e10.x, ex);
@@ -122,37 +271,44 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		ey = fma(w01, e01.y, ey);
 		ey = fma(w10, e10.y, ey);
 		ey = fma(w11, e11.y, ey);
-		double u = ux[i], v = uy[i];
-		boris_bz<double>(k.h[s], k.tz[s], k.sz[s], ex, ey, u, v);
-		if(fabs(u) > g.umax_x || fabs(v) > g.umax_y)
+		double uu = ux[i], vv = uy[i];
+		boris_bz<double>(k.h[s], k.tz[s], k.sz[s], ex, ey, uu, vv);
+		if(fabs(uu) > g.umax_x || fabs(vv) > g.umax_y)
 			bad = true;
 		if(set_r)
 		{
-			px = wrap64(fma(u, dtp, px), g.L0);
-			py = wrap64(fma(v, dtp, py), g.L1);
+			px = wrap64(fma(uu, dtp, px), g.L0);
+			py = wrap64(fma(vv, dtp, py), g.L1);
 		}
-		if(!(isfinite(px) && isfinite(py) && isfinite(u) && isfinite(v)))
+		if(!(isfinite(px) && isfinite(py) && isfinite(uu) && isfinite(vv)))
 		{
 			nan_ = true;
 			px = py = 0.0;
 		}
 		x[i] = px;
 		y[i] = py;
-		ux[i] = u;
-		uy[i] = v;
-		cic_nodes(g, px, py, i00, i01, i10, i11, rx, ry);
-		double c = k.c[s];
-		dxw = 1.0 - rx;
-		dyw = 1.0 - ry;
-		unsafeAtomicAdd(&rho_next[i00], (dxw * dyw) * c);
-		unsafeAtomicAdd(&rho_next[i01], (dxw * ry) * c);
-		unsafeAtomicAdd(&rho_next[i10], (rx * dyw) * c);
-		unsafeAtomicAdd(&rho_next[i11], (rx * ry) * c);
+		ux[i] = uu;
+		uy[i] = vv;
+		cell_of(g, px, py, ix[u], iy[u], rx[u], ry[u]);
+		lox = min(lox, ix[u]);
+		hix = max(hix, ix[u]);
+		loy = min(loy, iy[u]);
+		hiy = max(hiy, iy[u]);
 	}
 	if(bad)
 		atomicOr(derr, CPIC_DERR_UMAX);
 	if(nan_)
 		atomicOr(derr, CPIC_DERR_NAN);
+	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	box_zero(b, sacc, 1);
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
+	{
+		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
+		if(ix[u] >= 0)
+			dep1(g, b, sacc, rho_next, ix[u], iy[u], rx[u], ry[u], k.c[sp[i]]);
+	}
+	box_flush(g, b, sacc, 1, rho_next, 1);
 }
 
 // part[b] = sum over block b's contiguous range of a*b (fixed order)
@@ -203,9 +359,19 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double *__restrict__ ly, double *__restrict__ lux, double *__restrict__ luy,
 		double2 *__restrict__ lE)
 {
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
-			i += (int64_t) gridDim.x * blockDim.x)
+	__shared__ double sacc[2 * ADJ_BOX_NODES];
+	__shared__ int s4[4];
+	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
+	int ix[ADJ_PPT], iy[ADJ_PPT];
+	double rx[ADJ_PPT], ry[ADJ_PPT], lex[ADJ_PPT], ley[ADJ_PPT];
+	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
 	{
+		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
+		ix[u] = -1;
+		if(i >= n)
+			continue;
 		int s = sp[i];
 		double mx = lux[i], my = luy[i];
 		double ax = lx[i], ay = ly[i];
@@ -218,33 +384,57 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double a_ = 1.0 - tz * sz, b_ = sz;
 		double nux = a_ * mx - b_ * my;      // M^T mu
 		double nuy = b_ * mx + a_ * my;
-		double lex = h * (nux + mx), ley = h * (nuy + my);
+		lex[u] = h * (nux + mx);
+		ley[u] = h * (nuy + my);
 		lux[i] = nux;
 		luy[i] = nuy;
 
-		int i00, i01, i10, i11;
-		double rx, ry;
-		cic_nodes(g, x[i], y[i], i00, i01, i10, i11, rx, ry);
-		double dxw = 1.0 - rx, dyw = 1.0 - ry;
-		double2 e00 = E[i00], e01 = E[i01], e10 = E[i10], e11 = E[i11];
+		cell_of(g, x[i], y[i], ix[u], iy[u], rx[u], ry[u]);
+		int i1x = ix[u] + 1 == g.nx ? 0 : ix[u] + 1;
+		int i1y = iy[u] + 1 == g.ny ? 0 : iy[u] + 1;
+		double dxw = 1.0 - rx[u], dyw = 1.0 - ry[u];
+		double2 e00 = E[(int64_t) iy[u] * g.nx + ix[u]], e01 = E[(int64_t) i1y * g.nx + ix[u]];
+		double2 e10 = E[(int64_t) iy[u] * g.nx + i1x], e11 = E[(int64_t) i1y * g.nx + i1x];
 		// d/dx and d/dy of the CIC interpolants of Ex and Ey
-		double gxx = g.idx * (dyw * (e10.x - e00.x) + ry * (e11.x - e01.x));
-		double gyx = g.idy * (dxw * (e01.x - e00.x) + rx * (e11.x - e10.x));
-		double gxy = g.idx * (dyw * (e10.y - e00.y) + ry * (e11.y - e01.y));
-		double gyy = g.idy * (dxw * (e01.y - e00.y) + rx * (e11.y - e10.y));
-		lx[i] = ax + (lex * gxx + ley * gxy);
-		ly[i] = ay + (lex * gyx + ley * gyy);
-
-		double w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
-		unsafeAtomicAdd(&lE[i00].x, w00 * lex);
-		unsafeAtomicAdd(&lE[i00].y, w00 * ley);
-		unsafeAtomicAdd(&lE[i01].x, w01 * lex);
-		unsafeAtomicAdd(&lE[i01].y, w01 * ley);
-		unsafeAtomicAdd(&lE[i10].x, w10 * lex);
-		unsafeAtomicAdd(&lE[i10].y, w10 * ley);
-		unsafeAtomicAdd(&lE[i11].x, w11 * lex);
-		unsafeAtomicAdd(&lE[i11].y, w11 * ley);
+		double gxx = g.idx * (dyw * (e10.x - e00.x) + ry[u] * (e11.x - e01.x));
+		double gyx = g.idy * (dxw * (e01.x - e00.x) + rx[u] * (e11.x - e10.x));
+		double gxy = g.idx * (dyw * (e10.y - e00.y) + ry[u] * (e11.y - e01.y));
+		double gyy = g.idy * (dxw * (e01.y - e00.y) + rx[u] * (e11.y - e10.y));
+		lx[i] = ax + (lex[u] * gxx + ley[u] * gxy);
+		ly[i] = ay + (lex[u] * gyx + ley[u] * gyy);
+		lox = min(lox, ix[u]);
+		hix = max(hix, ix[u]);
+		loy = min(loy, iy[u]);
+		hiy = max(hiy, iy[u]);
 	}
+	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	box_zero(b, sacc, 2);
+#pragma unroll
+	for(int u = 0; u < ADJ_PPT; u++)
+	{
+		if(ix[u] < 0)
+			continue;
+		double dxw = 1.0 - rx[u], dyw = 1.0 - ry[u];
+		double w[4] = {dxw * dyw, dxw * ry[u], rx[u] * dyw, rx[u] * ry[u]};
+		const int ddx[4] = {0, 0, 1, 1}, ddy[4] = {0, 1, 0, 1};
+#pragma unroll
+		for(int c = 0; c < 4; c++)
+		{
+			if(b.lds)
+			{
+				int nd = box_node(b, ix[u], iy[u], ddx[c], ddy[c]);
+				atomicAdd(&sacc[2 * nd], w[c] * lex[u]);
+				atomicAdd(&sacc[2 * nd + 1], w[c] * ley[u]);
+			}
+			else
+			{
+				int64_t nd = wrap_node(g, ix[u] + ddx[c], iy[u] + ddy[c]);
+				unsafeAtomicAdd(&lE[nd].x, w[c] * lex[u]);
+				unsafeAtomicAdd(&lE[nd].y, w[c] * ley[u]);
+			}
+		}
+	}
+	box_flush(g, b, sacc, 2, (double *) lE, 2);
 }
 
 // psi = D^T lE - srho * rho, D the central-difference E stencil
@@ -290,6 +480,13 @@ void k_adj_rev_b(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ sp,
 
 // ---------------------------------------------------------------- host ---
 
+// blocks of the depositing kernels: one per ADJ_CHUNK particles
+static int
+chunks(int64_t n)
+{
+	return (int) ((n + ADJ_CHUNK - 1) / ADJ_CHUNK);
+}
+
 static int
 pblocks(int64_t n)
 {
@@ -311,6 +508,9 @@ adj_free(cpic_adj *a)
 	hipFree(a->E); hipFree(a->lE);
 	hipFree(a->lx); hipFree(a->ly); hipFree(a->lux); hipFree(a->luy);
 	hipFree(a->ck); hipFree(a->seg); hipFree(a->hist); hipFree(a->part); hipFree(a->derr);
+	free(a->hsp);
+	free(a->perm);
+	free(a->hbuf);
 	if(a->d)
 		cpic_dev_destroy(a->d);
 	free(a);
@@ -377,11 +577,15 @@ cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
 		adj_free(a);
 		return CPIC_ENOMEM;
 	}
-	if(hipMemcpy(a->sp, species, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess)
+	a->hsp = (int32_t *) malloc(sizeof(int32_t) * (size_t) n);
+	a->perm = (int64_t *) malloc(sizeof(int64_t) * (size_t) n);
+	a->hbuf = (double *) malloc(sizeof(double) * 4 * (size_t) n);
+	if(!a->hsp || !a->perm || !a->hbuf)
 	{
 		adj_free(a);
-		return CPIC_EHIP;
+		return CPIC_ENOMEM;
 	}
+	memcpy(a->hsp, species, sizeof(int32_t) * n);
 	*out = a;
 	return 0;
 }
@@ -422,7 +626,7 @@ deposit(cpic_adj *a, const double *x, const double *y, double *rho)
 	int ret;
 	if((ret = zero(a, rho, (int64_t) a->g.nx * a->g.ny)))
 		return ret;
-	hipLaunchKernelGGL(k_adj_deposit, dim3(pblocks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+	hipLaunchKernelGGL(k_adj_deposit, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
 			a->g, a->k[1], a->n, a->sp, x, y, rho);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -459,7 +663,7 @@ forward_iter(cpic_adj *a, int64_t it, int *cur, double *w)
 		return ret;
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
-	hipLaunchKernelGGL(k_adj_push, dim3(pblocks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+	hipLaunchKernelGGL(k_adj_push, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
 			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, a->x, a->y, a->ux,
 			a->uy, (const double2 *) a->E, a->rho[1 - c], a->derr);
 	HIPCHK(hipGetLastError());
@@ -495,6 +699,58 @@ copy_state(cpic_adj *a, double *dst, const double *x, const double *y, const dou
 	return 0;
 }
 
+// perm: the particles in the order of their cell (iy*nx + ix), counting
+// sort, stable; the deposits' LDS boxes rely on neighbours in this order
+// being neighbours on the grid.
+static int
+sort_by_cell(cpic_adj *a, const double *x, const double *y)
+{
+	const int64_t n = a->n, nc = (int64_t) a->g.nx * a->g.ny;
+	int64_t *start = (int64_t *) calloc((size_t) nc + 1, sizeof(int64_t));
+	int64_t *cell = (int64_t *) malloc(sizeof(int64_t) * (size_t) n);
+	if(!start || !cell)
+	{
+		free(start);
+		free(cell);
+		return CPIC_ENOMEM;
+	}
+	for(int64_t i = 0; i < n; i++)
+	{
+		int64_t ix = (int64_t) (x[i] * a->g.idx), iy = (int64_t) (y[i] * a->g.idy);
+		ix = ix < 0 ? 0 : (ix >= a->g.nx ? a->g.nx - 1 : ix);
+		iy = iy < 0 ? 0 : (iy >= a->g.ny ? a->g.ny - 1 : iy);
+		cell[i] = iy * a->g.nx + ix;
+		start[cell[i] + 1]++;
+	}
+	for(int64_t c = 0; c < nc; c++)
+		start[c + 1] += start[c];
+	for(int64_t i = 0; i < n; i++)
+		a->perm[start[cell[i]]++] = i;
+	free(start);
+	free(cell);
+	return 0;
+}
+
+// Device arrays in sorted order -> host arrays in the caller's order.
+static int
+download_unsorted(cpic_adj *a, double *const *src, double *const *dst)
+{
+	const int64_t n = a->n;
+	size_t pn = sizeof(double) * (size_t) n;
+	for(int c = 0; c < 4; c++)
+		if(dst[c])
+			HIPCHK(hipMemcpyAsync(a->hbuf + c * n, src[c], pn, hipMemcpyDeviceToHost, a->d->stream));
+	HIPCHK(hipStreamSynchronize(a->d->stream));
+	for(int c = 0; c < 4; c++)
+		if(dst[c])
+		{
+			const double *h = a->hbuf + c * n;
+			for(int64_t i = 0; i < n; i++)
+				dst[c][a->perm[i]] = h[i];
+		}
+	return 0;
+}
+
 extern "C" int
 cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *ux,
 		const double *uy, int64_t nsteps, int32_t checkpoint_every, double *loss,
@@ -511,11 +767,27 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	int64_t K = checkpoint_every, nck = (nsteps + K - 1) / K, n = a->n;
 	if((ret = grow(&a->ck, &a->ckcap, 4 * n * nck)) || (ret = grow(&a->hist, &a->histcap, nsteps)))
 		return ret;
+	if((ret = sort_by_cell(a, x, y)))
+		return ret;
 	size_t pn = sizeof(double) * (size_t) n;
-	HIPCHK(hipMemcpyAsync(a->x, x, pn, hipMemcpyHostToDevice, a->d->stream));
-	HIPCHK(hipMemcpyAsync(a->y, y, pn, hipMemcpyHostToDevice, a->d->stream));
-	HIPCHK(hipMemcpyAsync(a->ux, ux, pn, hipMemcpyHostToDevice, a->d->stream));
-	HIPCHK(hipMemcpyAsync(a->uy, uy, pn, hipMemcpyHostToDevice, a->d->stream));
+	const double *src[4] = {x, y, ux, uy};
+	double *dev[4] = {a->x, a->y, a->ux, a->uy};
+	int32_t *ssp = (int32_t *) malloc(sizeof(int32_t) * (size_t) n);
+	if(!ssp)
+		return CPIC_ENOMEM;
+	for(int64_t i = 0; i < n; i++)
+		ssp[i] = a->hsp[a->perm[i]];
+	hipError_t e = hipMemcpy(a->sp, ssp, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+	free(ssp);
+	if(e != hipSuccess)
+		return CPIC_EHIP;
+	for(int c = 0; c < 4; c++)
+	{
+		double *h = a->hbuf + c * n;
+		for(int64_t i = 0; i < n; i++)
+			h[i] = src[c][a->perm[i]];
+		HIPCHK(hipMemcpyAsync(dev[c], h, pn, hipMemcpyHostToDevice, a->d->stream));
+	}
 	HIPCHK(hipMemsetAsync(a->derr, 0, sizeof(int), a->d->stream));
 	a->have = 0;
 	if((ret = deposit(a, a->x, a->y, a->rho[0])))
@@ -560,14 +832,9 @@ cpic_adj_state(cpic_adj_t *a, double *x, double *y, double *ux, double *uy)
 	if(!a || !a->have)
 		return CPIC_EINVAL;
 	hipSetDevice(a->d->device);
-	size_t pn = sizeof(double) * (size_t) a->n;
 	double *src[4] = {a->x, a->y, a->ux, a->uy};
 	double *dst[4] = {x, y, ux, uy};
-	for(int k = 0; k < 4; k++)
-		if(dst[k])
-			HIPCHK(hipMemcpyAsync(dst[k], src[k], pn, hipMemcpyDeviceToHost, a->d->stream));
-	HIPCHK(hipStreamSynchronize(a->d->stream));
-	return 0;
+	return download_unsorted(a, src, dst);
 }
 
 extern "C" int
@@ -623,7 +890,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			if((ret = deposit(a, sx, sy, a->rho[0])) || (ret = fields(a, a->rho[0], NULL)) ||
 					(ret = zero(a, (double *) a->lE, 2 * nodes)))
 				break;
-			hipLaunchKernelGGL(k_adj_rev_a, dim3(pblocks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+			hipLaunchKernelGGL(k_adj_rev_a, dim3(chunks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
 					a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
 					(const double2 *) a->E, a->lx, a->ly, a->lux, a->luy, a->lE);
 			hipLaunchKernelGGL(k_adj_field, dim3(pblocks(nodes)), dim3(CPIC_BLOCK), 0,
@@ -650,12 +917,9 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	hipFree(wk);
 	if(ret)
 		return ret;
-	size_t pn = sizeof(double) * (size_t) n;
 	double *src[4] = {a->lx, a->ly, a->lux, a->luy};
 	double *dst[4] = {gx, gy, gux, guy};
-	for(int k = 0; k < 4; k++)
-		if(dst[k])
-			HIPCHK(hipMemcpyAsync(dst[k], src[k], pn, hipMemcpyDeviceToHost, a->d->stream));
-	HIPCHK(hipStreamSynchronize(a->d->stream));
+	if((ret = download_unsorted(a, src, dst)))
+		return ret;
 	return adj_derr(a);
 }

@@ -11,10 +11,12 @@
 //                                  in both modes: on gfx950 ds_add_f64 is cheap while
 //                                  ds_add_f32 costs more than the whole HBM stream
 //                                  (measured at 2048^2: 5.8 ms vs 2.1 ms per launch)
-//   and compacts the tile's stayers in place while leavers (particles whose
-//   cell moved to another tile: comm_plasma's collect, src/comm_plasma.c:
-//   101-165) are appended to a leaver buffer through one wave-aggregated
-//   atomic per wavefront (ballot + popcount).
+//   and writes the tile's stayers to the other particle buffer (the two
+//   buffers swap every step; each wavefront claims its stayers' slots with
+//   one LDS atomic, so no block-wide barrier is needed per chunk) while
+//   leavers (particles whose cell moved to another tile: comm_plasma's
+//   collect, src/comm_plasma.c:101-165) are appended to a leaver buffer
+//   through one wave-aggregated atomic per wavefront (ballot + popcount).
 // The LDS rho region is flushed once per tile with FP64 global atomics.
 // k_append then places each leaver at the end of its destination bucket
 // (inject_particles, src/comm_plasma.c:358-400) or, when that bucket is full,
@@ -50,8 +52,8 @@ struct PushArgs {
 	int P;
 	int ntiles;           // tiles of the whole batch
 	Buckets bk;
-	Parts p;              // particles read (and, in place, written)
-	Parts q;              // particles written when PUSH_PINGPONG
+	Parts p;              // particles read
+	Parts q;              // particles written (the other buffer)
 	Leavers lv;
 	int par;              // leaver counter parity
 	int first;            // iteration 0: dt' = -dt/2 and no position update
@@ -393,13 +395,6 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 	return true;
 }
 
-// Fused gather + push + wrap + deposit + in-place tile compaction.  Each
-// thread handles U particles per chunk (particle base + u*256 + tid), all
-// loads of a chunk issued before any of them is used, so a workgroup keeps
-// U*256 particles of HBM reads in flight.  Stayers are written in their
-// original order at the front of the bucket (their write index never
-// exceeds a read index of the same or a later chunk); leavers go to the
-// leaver buffer with one atomic per wavefront.
 // Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
 // thread tid takes base + u*CPIC_BLOCK + tid (coalesced SoA loads).
 template <typename T, bool IDS, int U>
@@ -422,13 +417,15 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #define PUSH_PREFETCH 0
 #endif
 
-// 1: k_push reads one particle buffer and writes the other (the two swap
-// every step); each wavefront then claims its stayers' slots with one LDS
-// atomic instead of a block-wide scan and barrier per chunk
-#ifndef PUSH_PINGPONG
-#define PUSH_PINGPONG 0
-#endif
 
+// Fused gather + push + wrap + deposit + tile compaction.  Each thread
+// handles U particles per chunk (particle base + u*256 + tid), all loads of
+// a chunk issued before any of them is used, so a workgroup keeps U*256
+// particles of HBM reads in flight.  Stayers go to the front of the same
+// bucket in the other buffer (a.q), each wavefront claiming its slots with
+// one LDS atomic (their order within the bucket is not kept); leavers go
+// to the leaver buffer with one atomic per wavefront.
+//
 // Waves per SIMD the register allocation of k_push must allow.  Measured at
 // 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (79 VGPRs with
 // U = 2), the FP64 kernel loses from any bound (its state is twice the size).
@@ -441,7 +438,6 @@ void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
 	extern __shared__ unsigned char smem[];
-	__shared__ int s_cnt[2][U][CPIC_BLOCK / 64];
 	__shared__ int s_kcount;
 	__shared__ int s_w;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
@@ -476,7 +472,6 @@ void k_push(PushArgs a)
 	__syncthreads();
 
 	double ke = 0.0, mpx = 0.0, mpy = 0.0;
-	int buf = 0;
 	bool bad = false, nan_ = false;
 
 	for(int s = 0; s < g.ns; s++)
@@ -488,7 +483,6 @@ void k_push(PushArgs a)
 		const int ntot = n1 + a.bk.n2[b];
 		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
 		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
-		int w = 0;
 
 #if PUSH_PREFETCH
 		PState<T> nq[U];
@@ -565,7 +559,6 @@ void k_push(PushArgs a)
 						atomicAdd(&s_kcount, __popcll(placed));
 				}
 			}
-#if PUSH_PINGPONG
 			// stayers: one LDS atomic per wavefront claims their slots in
 			// the other buffer (order within the bucket is not kept)
 #pragma unroll
@@ -582,47 +575,13 @@ void k_push(PushArgs a)
 						store_p<T>(a.q, bstart + wb + __popcll(smk & lt), q[u], IDS, PE);
 				}
 			}
-#else
-			// stayers: block-wide stable compaction
-			unsigned long long sm[U];
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				sm[u] = __ballot(act[u] && !leave[u]);
-				if(lane == 0)
-					s_cnt[buf][u][wid] = __popcll(sm[u]);
-			}
-			__syncthreads();
-			int run = w;
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				int pre = 0, tot = 0;
-#pragma unroll
-				for(int k = 0; k < CPIC_BLOCK / 64; k++)
-				{
-					int v = s_cnt[buf][u][k];
-					pre += k < wid ? v : 0;
-					tot += v;
-				}
-				if(act[u] && !leave[u])
-					store_p<T>(a.p, bstart + run + pre + __popcll(sm[u] & lt), q[u], IDS, PE);
-				run += tot;
-			}
-			w = run;
-			buf ^= 1;
-#endif
 		}
 
 		__syncthreads();
 		if(tid == 0)
 		{
-#if PUSH_PINGPONG
 			a.bk.n1[b] = s_w;
 			s_w = 0;
-#else
-			a.bk.n1[b] = w;
-#endif
 			a.bk.res[b] = s_kcount;
 			a.bk.n2[b] = 0;
 			s_kcount = 0;
@@ -1247,9 +1206,7 @@ cpic_parts_push(cpic_dev *d, int first_iter)
 		DISPATCH_U(k_push, float, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
-#if PUSH_PINGPONG
 	d->pcur = 1 - d->pcur;
-#endif
 
 	slot = cpic_kt_begin(d, "append_leavers");
 	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->geo, PRE(d, PRE_BUCKET),
@@ -1465,12 +1422,10 @@ cpic_parts_rebuild(cpic_dev *d)
 		HIPCHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
 		HIPCHK(hipStreamSynchronize(d->stream));
 		d->pcur = spare;
-#if PUSH_PINGPONG
 		// the push writes the other buffer: it needs the same capacity
 		if(d->pcap[1 - spare] < d->pcap[spare])
 			if((ret = cpic_parts_alloc(d, 1 - spare, d->pcap[spare])))
 				goto out;
-#endif
 		// block order for the new counts
 		int *hc = (int *) malloc(sizeof(int) * (size_t) nb);
 		if(!hc)
