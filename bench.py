@@ -222,12 +222,18 @@ def run_ensemble(args, ca):
     missing = sorted({p["grid"] for p in plist if f"fp32/{p['grid']}" not in table})
     if missing and args.autotune:
         table = autotune.tune(missing, 32, log=log)
+    # one particles-per-thread setting per batch: that of the grid edge
+    # carrying the most particles
+    load = {}
+    for p in plist:
+        load[p["grid"]] = load.get(p["grid"], 0) + p["grid"] ** 2 * p["ppc"]
+    ppt = autotune.ppt_for(max(load, key=load.get), 32, table)
     descs = []
     for p in plist:
         g = p["grid"]
         descs.append(ca.make_desc(g, g, (float(g), float(g)), p["dt"], [-1.0, -1.0],
                                   [float(p["ppc"]), float(p["ppc"])], precision=32,
-                                  tile=autotune.tile_for(g, 32, table), flags=0))
+                                  tile=autotune.tile_for(g, 32, table), flags=0, ppt=ppt))
     t0 = time.time()
     b = ca.Batch(descs)
     empty = np.empty(0, np.int64)
@@ -275,6 +281,7 @@ def run_ensemble(args, ca):
                                f"captured-graph replay",
                    "plasmas": len(plist), "particles": ntot, "nodes": nodes,
                    "tiles": {str(g): list(autotune.tile_for(g, 32, table)) for g in ENSEMBLE_GRIDS},
+                   "ppt": ppt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
@@ -407,14 +414,20 @@ def main():
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
             log(f"   {k:24s} {n:6d} launches {ms / max(n, 1):9.4f} ms avg")
         hist = None
+        snaps = {}
         if args.accuracy_steps > 0:
+            # untimed, same seed in both precisions: rho and E snapshots at
+            # fixed iterations, the field-energy history up to accuracy_steps
             done = args.warmup + 2 * steps
-            if args.accuracy_steps > done:
-                d.run(done, args.accuracy_steps - done)   # untimed, same seed in both precisions
-            d.sync()
+            for at in sorted({max(done, 200), args.accuracy_steps}):
+                if at > done:
+                    d.run(done, at - done)
+                    done = at
+                d.sync()
+                snaps[done] = {f: d.field(f) for f in ("rho", "Ex")}
             hist = d.energy()["field"][:args.accuracy_steps]
         res[prec] = dict(elapsed=el, elapsed_graph=el_graph, steps=steps, kt=kt, stats=st,
-                         hist=hist)
+                         hist=hist, snaps=snaps)
         d.close()
         del d
 
@@ -470,6 +483,17 @@ def main():
                 "max_rel_first_100": float(rel[:100].max()),
                 "first_step_above_1e-5": int(above[0]) if len(above) else None,
                 "energy_growth": float(h64[n - 1] / h64[0]),
+            }
+        common = sorted(set(res[32]["snaps"]) & set(r64["snaps"]))
+        if common:
+            def rl2(a, b):
+                return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            out["fp32_field_error"] = {
+                "reference": "FP64 GPU run of the same seed",
+                "rho_rel_l2": {str(k): rl2(res[32]["snaps"][k]["rho"], r64["snaps"][k]["rho"])
+                               for k in common},
+                "Ex_rel_l2": {str(k): rl2(res[32]["snaps"][k]["Ex"], r64["snaps"][k]["Ex"])
+                              for k in common},
             }
     if not args.no_cpu:
         try:

@@ -45,7 +45,7 @@ class Desc(C.Structure):
         ("nchunks", C.c_int64),
         ("tile", C.c_int32 * 2), ("flags", C.c_int32),
         ("slack", C.c_double),
-        ("device", C.c_int32), ("reserved", C.c_int32),
+        ("device", C.c_int32), ("ppt", C.c_int32),
     ]
 
 
@@ -145,7 +145,7 @@ def _ip(a):
 
 
 def make_desc(nx, ny, L, dt, q, m, e0=1.0, B=(0.0, 0.0, 0.0), nchunks=1, precision=64,
-              tile=(0, 0), flags=0, slack=0.0, device=0):
+              tile=(0, 0), flags=0, slack=0.0, device=0, ppt=0):
     """A cpic_dev_desc_t from the reference's configuration values."""
     desc = Desc()
     desc.nx, desc.ny = int(nx), int(ny)
@@ -162,6 +162,7 @@ def make_desc(nx, ny, L, dt, q, m, e0=1.0, B=(0.0, 0.0, 0.0), nchunks=1, precisi
     desc.flags = int(flags)
     desc.slack = float(slack)
     desc.device = int(device)
+    desc.ppt = int(ppt)
     return desc
 
 
@@ -172,9 +173,9 @@ class Device:
     stage calls act on every plasma of the handle."""
 
     def __init__(self, nx, ny, L, dt, q, m, e0=1.0, B=(0.0, 0.0, 0.0), nchunks=1,
-                 precision=64, tile=(0, 0), flags=0, slack=0.0, device=0):
+                 precision=64, tile=(0, 0), flags=0, slack=0.0, device=0, ppt=0):
         desc = make_desc(nx, ny, L, dt, q, m, e0, B, nchunks, precision, tile, flags, slack,
-                         device)
+                         device, ppt)
         h = C.c_void_p()
         check(lib().cpic_dev_create(C.byref(desc), C.byref(h)))
         self._init([desc], h)

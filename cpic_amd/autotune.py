@@ -1,8 +1,9 @@
-"""Tile-shape autotuner for batched runs.
+"""Tile-shape and particles-per-thread autotuner for batched runs.
 
 For each grid edge and precision, times the full step (solve + push +
 migration + deposit, captured-graph replay) of a batch of identical
-two-stream plasmas under every candidate tile shape and keeps the fastest.
+two-stream plasmas under every candidate (tile shape, particles per thread
+and chunk of the push kernel) and keeps the fastest.
 Results are cached as JSON next to this file (tuned/tiles.json), keyed by
 "fp<precision>/<grid edge>", with the measured rates, so a later run (or
 another process) reuses them without timing again.
@@ -24,8 +25,10 @@ import numpy as np
 import cpic_amd as ca
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "tiles.json")
-CANDIDATES = ((8, 8), (16, 8), (8, 16), (16, 16), (32, 16), (16, 32), (32, 32))
+CANDIDATES = ((8, 8), (16, 8), (16, 16), (32, 16), (16, 32), (32, 32))
+PPT = (2, 4)
 DEFAULT_TILE = (16, 16)
+DEFAULT_PPT = 2
 
 
 def two_stream(rng, grid, ppc, v0, sigma):
@@ -57,9 +60,16 @@ def tile_for(grid, precision, table=None):
     return tuple(e["tile"]) if e else DEFAULT_TILE
 
 
-def time_candidate(grid, tile, precision, parts, m, nplasmas, steps, warmup, dt=0.1):
+def ppt_for(grid, precision, table=None):
+    """Tuned particles per thread for a grid edge (the default when not tuned)."""
+    table = load() if table is None else table
+    e = table.get(f"fp{precision}/{grid}")
+    return int(e.get("ppt", DEFAULT_PPT)) if e else DEFAULT_PPT
+
+
+def time_candidate(grid, tile, precision, parts, m, nplasmas, steps, warmup, dt=0.1, ppt=0):
     descs = [ca.make_desc(grid, grid, (float(grid), float(grid)), dt, [-1.0, -1.0], [m, m],
-                          precision=precision, tile=tile, flags=ca.F_NO_DIAG)
+                          precision=precision, tile=tile, flags=ca.F_NO_DIAG, ppt=ppt)
              for _ in range(nplasmas)]
     b = ca.Batch(descs)
     empty = np.empty(0, np.int64)
@@ -89,13 +99,17 @@ def tune(grids, precision=32, ppc=64, target_particles=1 << 25, steps=20, warmup
         for tile in candidates:
             if tile[0] > grid or tile[1] > grid:
                 continue
-            r = time_candidate(grid, tile, precision, parts, m, nplasmas, steps, warmup)
-            rates[f"{tile[0]}x{tile[1]}"] = r
-            log(f"autotune fp{precision} grid {grid} x{nplasmas} tile {tile}: {r:.4g} particle-steps/s")
+            for ppt in PPT:
+                r = time_candidate(grid, tile, precision, parts, m, nplasmas, steps, warmup,
+                                   ppt=ppt)
+                rates[f"{tile[0]}x{tile[1]}/{ppt}"] = r
+                log(f"autotune fp{precision} grid {grid} x{nplasmas} tile {tile} ppt {ppt}: "
+                    f"{r:.4g} particle-steps/s")
         best = max(rates, key=rates.get)
+        shape, ppt = best.split("/")
         table[f"fp{precision}/{grid}"] = {
-            "tile": [int(v) for v in best.split("x")], "ppc": ppc, "plasmas": nplasmas,
-            "rates": rates,
+            "tile": [int(v) for v in shape.split("x")], "ppt": int(ppt), "ppc": ppc,
+            "plasmas": nplasmas, "rates": rates,
         }
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:

@@ -226,6 +226,8 @@ desc_check(const cpic_dev_desc_t *desc)
 		return CPIC_EINVAL;
 	if(desc->precision != 32 && desc->precision != 64)
 		return CPIC_EINVAL;
+	if(desc->ppt != 0 && desc->ppt != 2 && desc->ppt != 4)
+		return CPIC_EINVAL;
 	// In-plane B makes u_z non-zero, which the reference rejects through
 	// umax_Z = 0 (src/sim.c:200, src/mover.c:113).
 	if(desc->B[0] != 0.0 || desc->B[1] != 0.0)
@@ -306,7 +308,8 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 			return ret;
 		// one precision, flag set, device and slack for the whole batch
 		if(descs[p].precision != descs[0].precision || descs[p].flags != descs[0].flags ||
-				descs[p].device != descs[0].device || descs[p].slack != descs[0].slack)
+				descs[p].device != descs[0].device || descs[p].slack != descs[0].slack ||
+				descs[p].ppt != descs[0].ppt)
 			return CPIC_EINVAL;
 	}
 	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -321,6 +324,7 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 		return CPIC_ENOMEM;
 	d->P = P;
 	d->prec = descs[0].precision;
+	d->ppt = descs[0].ppt == 4 ? 4 : 2;
 	d->flags = descs[0].flags;
 	d->device = descs[0].device;
 	d->ring = P <= 4 ? 65536 : (262144 / P > 4096 ? 262144 / P : 4096);

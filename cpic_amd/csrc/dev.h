@@ -126,6 +126,7 @@ struct cpic_dev {
 	Geo *geo;                // [P] device
 	int *hpre, *pre;         // [PRE_COUNT][P+1] host / device
 	int prec;                // 32 or 64
+	int ppt;                 // particles per thread and chunk in the push: 2 or 4
 	int flags;
 	int device;
 	int maxns;

@@ -127,6 +127,31 @@ static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
 #define PUSH_NT_STORE 0
 #endif
 
+#define LDS_AS __attribute__((address_space(3)))
+#define GLB_AS __attribute__((address_space(1)))
+
+// A pointer into device memory as a global-address-space pointer.
+template <typename T>
+static __device__ __forceinline__ GLB_AS T *gptr(const void *p)
+{
+	return (GLB_AS T *) (T *) p;
+}
+
+// FP64 atomic adds with the address space spelled out, so the LDS and the
+// global paths of one call site are never merged into a generic (flat) op.
+static __device__ __forceinline__ void lds_add(LDS_AS double *p, double v)
+{
+	__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+static __device__ __forceinline__ void glb_add(GLB_AS double *p, double v)
+{
+	[[clang::atomic(ignore_denormal_mode)]]
+	{
+		__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
 // Per-particle state in registers.
 template <typename T> struct PState {
 	T x, y, ux, uy;
@@ -138,12 +163,14 @@ template <typename T>
 static __device__ __forceinline__ void load_p(const Parts &p, int64_t s, PState<T> &q,
 		bool ids)
 {
-	q.x = ((const T *) p.x)[s];
-	q.y = ((const T *) p.y)[s];
-	q.ux = ((const T *) p.ux)[s];
-	q.uy = ((const T *) p.uy)[s];
+	// global-address-space loads: generic (flat) loads would also count
+	// against the LDS wait counter and hold up the LDS gather behind them
+	q.x = gptr<T>(p.x)[s];
+	q.y = gptr<T>(p.y)[s];
+	q.ux = gptr<T>(p.ux)[s];
+	q.uy = gptr<T>(p.uy)[s];
 	if(ids)
-		q.id = p.id[s];
+		q.id = gptr<int64_t>(p.id)[s];
 }
 
 template <typename T>
@@ -209,22 +236,37 @@ static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>
 	T dxw = (T) 1 - rx, dyw = (T) 1 - ry;
 	T w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
 	T2 e00, e01, e10, e11;
+	// Explicit address spaces: with generic pointers the compiler merges the
+	// two branches into one set of flat loads, which go through the vector
+	// memory path even for LDS and wait for every outstanding HBM access.
 	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
 	{
+		const LDS_AS T *l = (const LDS_AS T *) sE;     // (x, y) pairs
 		int i = c.ly * g.W + c.lx;
-		e00 = sE[i];
-		e01 = sE[i + g.W];
-		e10 = sE[i + 1];
-		e11 = sE[i + g.W + 1];
+		e00.x = l[2 * i];
+		e00.y = l[2 * i + 1];
+		e01.x = l[2 * (i + g.W)];
+		e01.y = l[2 * (i + g.W) + 1];
+		e10.x = l[2 * (i + 1)];
+		e10.y = l[2 * (i + 1) + 1];
+		e11.x = l[2 * (i + g.W + 1)];
+		e11.y = l[2 * (i + g.W + 1) + 1];
 	}
 	else
 	{
+		const GLB_AS T *m = (const GLB_AS T *) E;
 		int i1x = c.ix + 1 == g.nx ? 0 : c.ix + 1;
 		int i1y = c.iy + 1 == g.ny ? 0 : c.iy + 1;
-		e00 = E[(int64_t) c.iy * g.nx + c.ix];
-		e01 = E[(int64_t) i1y * g.nx + c.ix];
-		e10 = E[(int64_t) c.iy * g.nx + i1x];
-		e11 = E[(int64_t) i1y * g.nx + i1x];
+		int64_t n00 = (int64_t) c.iy * g.nx + c.ix, n01 = (int64_t) i1y * g.nx + c.ix;
+		int64_t n10 = (int64_t) c.iy * g.nx + i1x, n11 = (int64_t) i1y * g.nx + i1x;
+		e00.x = m[2 * n00];
+		e00.y = m[2 * n00 + 1];
+		e01.x = m[2 * n01];
+		e01.y = m[2 * n01 + 1];
+		e10.x = m[2 * n10];
+		e10.y = m[2 * n10 + 1];
+		e11.x = m[2 * n11];
+		e11.y = m[2 * n11 + 1];
 	}
 	ex = w00 * e00.x;
 	ex = fma(w01, e01.x, ex);
@@ -245,20 +287,22 @@ static __device__ __forceinline__ void deposit(const Geo &g, double *sR, double 
 	T w00 = dxw * dyw, w01 = dxw * ry, w10 = rx * dyw, w11 = rx * ry;
 	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
 	{
+		LDS_AS double *l = (LDS_AS double *) sR;
 		int i = c.ly * g.W + c.lx;
-		atomicAdd(&sR[i], (double) (w00 * vq));
-		atomicAdd(&sR[i + g.W], (double) (w01 * vq));
-		atomicAdd(&sR[i + 1], (double) (w10 * vq));
-		atomicAdd(&sR[i + g.W + 1], (double) (w11 * vq));
+		lds_add(&l[i], (double) (w00 * vq));
+		lds_add(&l[i + g.W], (double) (w01 * vq));
+		lds_add(&l[i + 1], (double) (w10 * vq));
+		lds_add(&l[i + g.W + 1], (double) (w11 * vq));
 	}
 	else
 	{
+		GLB_AS double *m = gptr<double>(rho);
 		int i1x = c.ix + 1 == g.nx ? 0 : c.ix + 1;
 		int i1y = c.iy + 1 == g.ny ? 0 : c.iy + 1;
-		unsafeAtomicAdd(&rho[(int64_t) c.iy * g.nx + c.ix], (double) (w00 * vq));
-		unsafeAtomicAdd(&rho[(int64_t) i1y * g.nx + c.ix], (double) (w01 * vq));
-		unsafeAtomicAdd(&rho[(int64_t) c.iy * g.nx + i1x], (double) (w10 * vq));
-		unsafeAtomicAdd(&rho[(int64_t) i1y * g.nx + i1x], (double) (w11 * vq));
+		glb_add(&m[(int64_t) c.iy * g.nx + c.ix], (double) (w00 * vq));
+		glb_add(&m[(int64_t) i1y * g.nx + c.ix], (double) (w01 * vq));
+		glb_add(&m[(int64_t) c.iy * g.nx + i1x], (double) (w10 * vq));
+		glb_add(&m[(int64_t) i1y * g.nx + i1x], (double) (w11 * vq));
 	}
 }
 
@@ -433,7 +477,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #define PUSH_MIN_WAVES_F32 6
 #endif
 template <typename T, bool IDS, bool PE, int U>
-__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 ? PUSH_MIN_WAVES_F32 : 1)
+__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
@@ -528,7 +572,7 @@ void k_push(PushArgs a)
 					int lbase = 0;
 					if(lane == 0)
 						lbase = atomicAdd(&a.lv.count[a.par], lcnt);
-					lbase = __shfl(lbase, 0);
+					lbase = __builtin_amdgcn_readfirstlane(lbase);
 					if(leave[u])
 					{
 						int64_t ls = (int64_t) lbase + __popcll(lm & lt);
@@ -570,7 +614,7 @@ void k_push(PushArgs a)
 					int wb = 0;
 					if(lane == 0)
 						wb = atomicAdd(&s_w, __popcll(smk));
-					wb = __shfl(wb, 0);
+					wb = __builtin_amdgcn_readfirstlane(wb);
 					if(act[u] && !leave[u])
 						store_p<T>(a.q, bstart + wb + __popcll(smk & lt), q[u], IDS, PE);
 				}
@@ -696,11 +740,9 @@ void k_gather(PushArgs a)
 	}
 }
 
-// particles per thread and chunk in k_push (2 and 4 measure within 1-2 %;
-// 2 leaves room for the FP32 occupancy bound above)
-#ifndef PUSH_U
-#define PUSH_U 2
-#endif
+// particles per thread and chunk in k_push: 2 by default (2 and 4 measure
+// within 1-2 % at 2048^2; 2 leaves room for the FP32 occupancy bound above),
+// 4 on request (cpic_dev_desc_t.ppt, chosen by the autotuner)
 
 // Places leavers in their destination bucket, or back in the reserved
 // slots of their source bucket when the destination is full.  The last
@@ -1059,13 +1101,19 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 		else hipLaunchKernelGGL((kern<T, false, false>), __VA_ARGS__); \
 	} while(0)
 
-#define DISPATCH_U(kern, T, ...) do { \
+#define DISPATCH_UU(kern, T, U, ...) do { \
 		bool ids_ = (d->flags & CPIC_F_IDS) != 0; \
 		bool pe_ = (d->flags & CPIC_F_PARTICLE_E) != 0; \
-		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, PUSH_U>), __VA_ARGS__); \
-		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, PUSH_U>), __VA_ARGS__); \
-		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, PUSH_U>), __VA_ARGS__); \
-		else hipLaunchKernelGGL((kern<T, false, false, PUSH_U>), __VA_ARGS__); \
+		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, U>), __VA_ARGS__); \
+		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, U>), __VA_ARGS__); \
+		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, U>), __VA_ARGS__); \
+		else hipLaunchKernelGGL((kern<T, false, false, U>), __VA_ARGS__); \
+	} while(0)
+
+// particles per thread and chunk: 2 (default) or 4, chosen per handle
+#define DISPATCH_U(kern, T, ...) do { \
+		if(d->ppt == 4) DISPATCH_UU(kern, T, 4, __VA_ARGS__); \
+		else DISPATCH_UU(kern, T, 2, __VA_ARGS__); \
 	} while(0)
 
 #define DISPATCH_P(kern, ...) do { \
@@ -1556,10 +1604,14 @@ cpic_parts_init(cpic_dev *d)
 {
 	(void) d;
 	const void *k[] = {
-		(const void *) k_push<double, false, false, PUSH_U>, (const void *) k_push<double, true, false, PUSH_U>,
-		(const void *) k_push<double, false, true, PUSH_U>, (const void *) k_push<double, true, true, PUSH_U>,
-		(const void *) k_push<float, false, false, PUSH_U>, (const void *) k_push<float, true, false, PUSH_U>,
-		(const void *) k_push<float, false, true, PUSH_U>, (const void *) k_push<float, true, true, PUSH_U>,
+		(const void *) k_push<double, false, false, 2>, (const void *) k_push<double, true, false, 2>,
+		(const void *) k_push<double, false, true, 2>, (const void *) k_push<double, true, true, 2>,
+		(const void *) k_push<float, false, false, 2>, (const void *) k_push<float, true, false, 2>,
+		(const void *) k_push<float, false, true, 2>, (const void *) k_push<float, true, true, 2>,
+		(const void *) k_push<double, false, false, 4>, (const void *) k_push<double, true, false, 4>,
+		(const void *) k_push<double, false, true, 4>, (const void *) k_push<double, true, true, 4>,
+		(const void *) k_push<float, false, false, 4>, (const void *) k_push<float, true, false, 4>,
+		(const void *) k_push<float, false, true, 4>, (const void *) k_push<float, true, true, 4>,
 		(const void *) k_deposit<double>, (const void *) k_deposit<float>,
 		(const void *) k_gather<double>, (const void *) k_gather<float>,
 	};

@@ -69,7 +69,7 @@ typedef struct cpic_dev_desc {
 	int32_t flags;             /* CPIC_F_* */
 	double slack;              /* spare bucket capacity fraction; <= 0 = default */
 	int32_t device;            /* HIP device ordinal */
-	int32_t reserved;
+	int32_t ppt;               /* particles per thread and chunk in the push: 2 or 4; 0 = 2 */
 } cpic_dev_desc_t;
 
 /* Output of cpic_dev_download_particles.  n is the capacity of every
@@ -210,7 +210,7 @@ int cpic_dev_set_profiling(cpic_dev_t *dev, int32_t on);
  */
 
 /* sim_init's device part for nplasmas configurations at once.  They must
- * share precision, flags, device and slack. */
+ * share precision, flags, device, slack and ppt. */
 int cpic_batch_create(const cpic_dev_desc_t *descs, int32_t nplasmas, cpic_dev_t **out);
 int cpic_batch_size(cpic_dev_t *dev, int32_t *nplasmas);
 int cpic_batch_upload_particles(cpic_dev_t *dev, int32_t plasma, int32_t species,

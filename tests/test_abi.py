@@ -47,7 +47,8 @@ def test_invalid_descriptions_rejected_before_any_device_call():
 
 def test_invalid_batches_rejected_before_any_device_call():
     good = dict(nx=16, ny=16, L=(16.0, 16.0), dt=0.1, q=[-1.0], m=[1.0])
-    mixed = [dict(precision=32), dict(flags=ca.F_IDS), dict(slack=0.5), dict(nx=48)]
+    mixed = [dict(precision=32), dict(flags=ca.F_IDS), dict(slack=0.5), dict(nx=48), dict(ppt=4),
+             dict(ppt=3)]
     for kw in mixed:
         other = dict(good)
         other.update(kw)

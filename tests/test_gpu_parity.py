@@ -19,23 +19,23 @@ FLAGS = ca.F_IDS | ca.F_PARTICLE_E
 
 
 def make_pair(nx, ny, L, dt, q, m, parts, nchunks=1, precision=64, tile=(0, 0), flags=FLAGS,
-              B=(0.0, 0.0, 0.0), e0=1.0, slack=0.0):
+              B=(0.0, 0.0, 0.0), e0=1.0, slack=0.0, ppt=0):
     o = po.Oracle(nx, ny, L, dt, q, m, e0=e0, B=B, nchunks=nchunks)
     d = ca.Device(nx, ny, L, dt, q, m, e0=e0, B=B, nchunks=nchunks, precision=precision,
-                  tile=tile, flags=flags, slack=slack)
+                  tile=tile, flags=flags, slack=slack, ppt=ppt)
     for s, (ids, x, y, ux, uy) in enumerate(parts):
         o.add(s, ids, x, y, ux, uy)
         d.upload(s, ids, x, y, ux, uy)
     return o, d
 
 
-def c1(seed=1, n=5120, precision=64, tile=(0, 0), nchunks=1, slack=0.0, flags=FLAGS):
+def c1(seed=1, n=5120, precision=64, tile=(0, 0), nchunks=1, slack=0.0, flags=FLAGS, ppt=0):
     """64x64 two-stream: 2 x 5120 electrons, q=-1, m such that w_p = 1."""
     L = (64.0, 64.0)
     m = 2 * n / (L[0] * L[1])
     parts = two_stream(n, L, 0.1, 0.01, seed)
     return make_pair(64, 64, L, 0.1, [-1.0, -1.0], [m, m], parts, nchunks=nchunks,
-                     precision=precision, tile=tile, slack=slack, flags=flags)
+                     precision=precision, tile=tile, slack=slack, flags=flags, ppt=ppt)
 
 
 def assert_same_particles(o, d, ns, exact=True, tol=1e-12):
@@ -341,3 +341,15 @@ def test_c2_scale_fp64_matches_oracle():
     assert d.count(0) == n and d.count(1) == n
     assert np.all(np.isfinite(d.energy()["field"][:500]))
     print(f"500 steps {time.time() - t0:.1f} s", flush=True)
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-5)])
+def test_four_particles_per_thread(precision, tol):
+    """The push kernel's ppt = 4 instantiation (autotuner choice) gives the
+    same results as the default."""
+    o, d = c1(seed=12, precision=precision, ppt=4, tile=(8, 8))
+    W = run_both(o, d, 20)
+    d.sync()
+    assert rel_l2(d.field("rho"), o.field("rho")) < tol
+    assert_same_particles(o, d, 2, exact=False, tol=tol)
+    assert rel_l2(d.energy()["field"], W) < max(tol, 1e-12) * 10
