@@ -77,8 +77,13 @@ struct Buckets {
 	int32_t *n2;
 };
 
+// Particle state (double or float) in groups of PGROUP particles: group k
+// holds x[64] y[64] ux[64] uy[64] of slots 64k..64k+63, so a wavefront's
+// four loads of 64 consecutive slots fall in one 1 KiB (FP32) or 2 KiB
+// (FP64) run of memory instead of four separate arrays.
+#define PGROUP 64
 struct Parts {
-	void *x, *y, *ux, *uy;   // double or float
+	void *r;                 // 4 * capacity elements, capacity a multiple of 64
 	int64_t *id;             // nullable
 	double *Ex, *Ey;         // nullable
 };

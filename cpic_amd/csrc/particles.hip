@@ -159,16 +159,37 @@ template <typename T> struct PState {
 	double Ex, Ey;
 };
 
+// Element offset of slot s's x in the grouped state; y, ux and uy follow
+// at +64, +128 and +192.
+static_assert(PGROUP == 64, "pslot assumes groups of 64");
+static __device__ __forceinline__ int64_t pslot(int64_t s)
+{
+	return ((s >> 6) << 8) + (s & 63);
+}
+
+template <typename T>
+static __device__ __forceinline__ T p_x(const Parts &p, int64_t s)
+{
+	return gptr<T>(p.r)[pslot(s)];
+}
+
+template <typename T>
+static __device__ __forceinline__ T p_y(const Parts &p, int64_t s)
+{
+	return gptr<T>(p.r)[pslot(s) + PGROUP];
+}
+
 template <typename T>
 static __device__ __forceinline__ void load_p(const Parts &p, int64_t s, PState<T> &q,
 		bool ids)
 {
 	// global-address-space loads: generic (flat) loads would also count
 	// against the LDS wait counter and hold up the LDS gather behind them
-	q.x = gptr<T>(p.x)[s];
-	q.y = gptr<T>(p.y)[s];
-	q.ux = gptr<T>(p.ux)[s];
-	q.uy = gptr<T>(p.uy)[s];
+	const GLB_AS T *r = gptr<T>(p.r) + pslot(s);
+	q.x = r[0];
+	q.y = r[PGROUP];
+	q.ux = r[2 * PGROUP];
+	q.uy = r[3 * PGROUP];
 	if(ids)
 		q.id = gptr<int64_t>(p.id)[s];
 }
@@ -177,17 +198,18 @@ template <typename T>
 static __device__ __forceinline__ void store_p(const Parts &p, int64_t s, const PState<T> &q,
 		bool ids, bool pe)
 {
+	GLB_AS T *r = gptr<T>(p.r) + pslot(s);
 #if PUSH_NT_STORE
 	// streaming stores: the state is read back only in the next step
-	__builtin_nontemporal_store(q.x, &((T *) p.x)[s]);
-	__builtin_nontemporal_store(q.y, &((T *) p.y)[s]);
-	__builtin_nontemporal_store(q.ux, &((T *) p.ux)[s]);
-	__builtin_nontemporal_store(q.uy, &((T *) p.uy)[s]);
+	__builtin_nontemporal_store(q.x, &r[0]);
+	__builtin_nontemporal_store(q.y, &r[PGROUP]);
+	__builtin_nontemporal_store(q.ux, &r[2 * PGROUP]);
+	__builtin_nontemporal_store(q.uy, &r[3 * PGROUP]);
 #else
-	((T *) p.x)[s] = q.x;
-	((T *) p.y)[s] = q.y;
-	((T *) p.ux)[s] = q.ux;
-	((T *) p.uy)[s] = q.uy;
+	r[0] = q.x;
+	r[PGROUP] = q.y;
+	r[2 * PGROUP] = q.ux;
+	r[3 * PGROUP] = q.uy;
 #endif
 	if(ids)
 		p.id[s] = q.id;
@@ -699,7 +721,7 @@ void k_deposit(PushArgs a)
 			int64_t slot = bucket_slot(a.bk, b, n1, j);
 			CellPos c;
 			T rx, ry;
-			locate(g, ox, oy, ((const T *) a.p.x)[slot], ((const T *) a.p.y)[slot], c, rx, ry);
+			locate(g, ox, oy, p_x<T>(a.p, slot), p_y<T>(a.p, slot), c, rx, ry);
 			deposit<T>(g, sR, rho, c, rx, ry, vq);
 		}
 	}
@@ -732,7 +754,7 @@ void k_gather(PushArgs a)
 			int64_t slot = bucket_slot(a.bk, b, n1, j);
 			CellPos c;
 			T rx, ry, ex, ey;
-			locate(g, ox, oy, ((const T *) a.p.x)[slot], ((const T *) a.p.y)[slot], c, rx, ry);
+			locate(g, ox, oy, p_x<T>(a.p, slot), p_y<T>(a.p, slot), c, rx, ry);
 			gather<T>(g, sE, E, c, rx, ry, ex, ey);
 			a.p.Ex[slot] = (double) ex;
 			a.p.Ey[slot] = (double) ey;
@@ -846,7 +868,7 @@ void k_hist_layout(const Geo *__restrict__ geo, const int *__restrict__ pre_buck
 		int64_t slot = bucket_slot(bk, b, n1, j);
 		CellPos c;
 		T rx, ry;
-		locate(g, ox, oy, ((const T *) p.x)[slot], ((const T *) p.y)[slot], c, rx, ry);
+		locate(g, ox, oy, p_x<T>(p, slot), p_y<T>(p, slot), c, rx, ry);
 		int nt = (c.iy >> g.lgty) * g.ntx + (c.ix >> g.lgtx);
 		atomicAdd(&cnt[g.bucket0 + nt * g.ns + s], 1);
 	}
@@ -1044,7 +1066,9 @@ void k_download(Geo g, Buckets bk, Parts p, int s, const int64_t *__restrict__ o
 	{
 		int64_t slot = bucket_slot(bk, b, n1, j);
 		int64_t o = off[tile] + j;
-		T x = ((const T *) p.x)[slot], y = ((const T *) p.y)[slot];
+		PState<T> q;
+		load_p<T>(p, slot, q, false);
+		T x = q.x, y = q.y;
 		CellPos c;
 		T rx, ry;
 		locate(g, ox, oy, x, y, c, rx, ry);
@@ -1061,8 +1085,8 @@ void k_download(Geo g, Buckets bk, Parts p, int s, const int64_t *__restrict__ o
 		}
 		out[o] = ax;
 		out[n + o] = ay;
-		out[2 * n + o] = (double) ((const T *) p.ux)[slot];
-		out[3 * n + o] = (double) ((const T *) p.uy)[slot];
+		out[2 * n + o] = (double) q.ux;
+		out[3 * n + o] = (double) q.uy;
 		out[4 * n + o] = PE ? p.Ex[slot] : 0.0;
 		out[5 * n + o] = PE ? p.Ey[slot] : 0.0;
 		oid[o] = IDS ? p.id[slot] : -1;
@@ -1084,7 +1108,7 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 		int64_t slot = bucket_slot(bk, b, n1, j);
 		CellPos c;
 		T rx, ry;
-		locate(g, ox, oy, ((const T *) p.x)[slot], ((const T *) p.y)[slot], c, rx, ry);
+		locate(g, ox, oy, p_x<T>(p, slot), p_y<T>(p, slot), c, rx, ry);
 		atomicAdd(&counts[(int64_t) c.iy * g.nx + c.ix], 1ull);
 	}
 }
@@ -1155,7 +1179,7 @@ void
 cpic_parts_free(cpic_dev *d, int which)
 {
 	Parts *p = &d->p[which];
-	hipFree(p->x); hipFree(p->y); hipFree(p->ux); hipFree(p->uy);
+	hipFree(p->r);
 	hipFree(p->id); hipFree(p->Ex); hipFree(p->Ey);
 	memset(p, 0, sizeof(*p));
 	d->pcap[which] = 0;
@@ -1166,12 +1190,9 @@ cpic_parts_alloc(cpic_dev *d, int which, int64_t cap)
 {
 	Parts *p = &d->p[which];
 	size_t e = elem(d);
-	if(cap < 64)
-		cap = 64;
+	cap = (cap < PGROUP ? PGROUP : cap + PGROUP - 1) / PGROUP * PGROUP;
 	cpic_parts_free(d, which);
-	if(hipMalloc(&p->x, e * cap) != hipSuccess || hipMalloc(&p->y, e * cap) != hipSuccess ||
-			hipMalloc(&p->ux, e * cap) != hipSuccess ||
-			hipMalloc(&p->uy, e * cap) != hipSuccess)
+	if(hipMalloc(&p->r, 4 * e * cap) != hipSuccess)
 		goto fail;
 	if((d->flags & CPIC_F_IDS) && hipMalloc(&p->id, sizeof(int64_t) * cap) != hipSuccess)
 		goto fail;
