@@ -64,9 +64,9 @@ def workload(grid, ppc):
                 sigma=(0.01, 0.01 / np.sqrt(1836.0)), dt=0.05)
 
 
-def make_device(ca, w, precision, flags, seed=2024, tile=(0, 0)):
+def make_device(ca, w, precision, flags, seed=2024, tile=(0, 0), ppt=0):
     d = ca.Device(w["grid"], w["grid"], (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]),
-                  nchunks=1, precision=precision, flags=flags, tile=tile)
+                  nchunks=1, precision=precision, flags=flags, tile=tile, ppt=ppt)
     for s in range(2):
         rng = np.random.default_rng([seed, s])
         x, y, ux, uy = gen_species(rng, w["n"][s], w["L"], w["v0"][s], w["sigma"][s])
@@ -362,7 +362,10 @@ def main():
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=50, help="0 skips the FP64 comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--tile", type=int, default=0, help="tile edge in cells (0 = library default)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="tile edge in cells (0 = the tuned shape for the grid, else the default)")
+    ap.add_argument("--ppt", type=int, default=0, choices=(0, 2, 4),
+                    help="push particles per thread (0 = tuned for the grid, else the default)")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps over which FP32 and FP64 field energies are compared (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -395,12 +398,15 @@ def main():
     ntot = sum(w["n"])
     ng = args.grid * args.grid
 
+    from cpic_amd import autotune
     res = {}
     for prec, steps in ((32, args.steps), (64, args.fp64_steps)):
         if steps <= 0:
             continue
         t0 = time.time()
-        d = make_device(ca, w, prec, ca.F_PROFILE, tile=(args.tile, args.tile))
+        tile = (args.tile, args.tile) if args.tile else autotune.tile_for(args.grid, prec)
+        ppt = args.ppt or (0 if args.tile else autotune.ppt_for(args.grid, prec))
+        d = make_device(ca, w, prec, ca.F_PROFILE, tile=tile, ppt=ppt)
         log(f"fp{prec}: init {time.time() - t0:.1f} s")
         el, kt = time_steps(d, 0, args.warmup, steps)
         st = d.stats()
@@ -427,7 +433,7 @@ def main():
                 snaps[done] = {f: d.field(f) for f in ("rho", "Ex")}
             hist = d.energy()["field"][:args.accuracy_steps]
         res[prec] = dict(elapsed=el, elapsed_graph=el_graph, steps=steps, kt=kt, stats=st,
-                         hist=hist, snaps=snaps)
+                         hist=hist, snaps=snaps, ppt=ppt or 2)
         d.close()
         del d
 
@@ -455,7 +461,8 @@ def main():
         "config": {"workload": f"two-stream {args.grid}^2, {args.ppc} ppc, {ntot} particles, "
                                f"electrons + ions (1:1836), FP32 particles + FP64 FFT Poisson",
                    "grid": [args.grid, args.grid], "particles": ntot, "ppc": args.ppc,
-                   "dt": w["dt"], "tile": r32["stats"]["tile"], "parallelism": "single GPU"},
+                   "dt": w["dt"], "tile": r32["stats"]["tile"], "ppt": r32["ppt"],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
                      "traffic": traffic,
@@ -470,7 +477,8 @@ def main():
         n64, ms64 = r64["kt"].get("push_gather_deposit", (0, 0.0))
         a64 = ms64 / max(n64, 1) / 1e3
         out["fp64"] = {"value": v64, "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
-                       "push_achieved_GBs": (ntot * 64 + ng * 24) / a64 / 1e9 if a64 else None}
+                       "push_achieved_GBs": (ntot * 64 + ng * 24) / a64 / 1e9 if a64 else None,
+                       "tile": r64["stats"]["tile"], "ppt": r64["ppt"]}
         out["fp32_over_fp64"] = value / v64
         h32, h64 = res[32]["hist"], r64["hist"]
         if h32 is not None and h64 is not None and len(h32) and len(h64):

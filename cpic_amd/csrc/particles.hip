@@ -498,6 +498,44 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #ifndef PUSH_MIN_WAVES_F32
 #define PUSH_MIN_WAVES_F32 6
 #endif
+
+// Leavers of one bucket wait in LDS and are placed in the leaver buffer with
+// one global atomic per bucket: a counter shared by every workgroup, hit once
+// per wavefront with a leaver, serialises in one L2 channel and stalls the
+// wavefront until it returns.  A wavefront whose leavers no longer fit in the
+// stage claims its slots with its own global atomic.
+#ifndef LV_STAGE
+#define LV_STAGE 256
+#endif
+template <typename T, bool IDS, bool PE> struct LvStage {
+	T x[LV_STAGE], y[LV_STAGE], ux[LV_STAGE], uy[LV_STAGE];
+	int dst[LV_STAGE];
+	int64_t id[IDS ? LV_STAGE : 1];
+	double Ex[PE ? LV_STAGE : 1], Ey[PE ? LV_STAGE : 1];
+	int n;        // records claimed (may pass LV_STAGE)
+	int cut;      // first record of the wavefront that did not fit
+	int base;     // first leaver-buffer slot of the staged records
+};
+
+// An FP32 leaver (coordinates relative to destination bucket ndst) put back
+// relative to its source tile at origin (ox, oy) + margin.
+template <typename T>
+static __device__ __forceinline__ void back_to_tile(const Geo &g, int ndst, int ox, int oy,
+		PState<T> &q)
+{
+	if(sizeof(T) != 4)
+		return;
+	int dt_ = (ndst - g.bucket0) / g.ns;
+	int dtx = (dt_ & (g.ntx - 1)) * g.tx - (ox + g.M);
+	int dty = (dt_ >> g.lgntx) * g.ty - (oy + g.M);
+	if(dtx > g.nx / 2) dtx -= g.nx;
+	if(dtx < -g.nx / 2) dtx += g.nx;
+	if(dty > g.ny / 2) dty -= g.ny;
+	if(dty < -g.ny / 2) dty += g.ny;
+	q.x = (T) ((float) q.x + (float) dtx);
+	q.y = (T) ((float) q.y + (float) dty);
+}
+
 template <typename T, bool IDS, bool PE, int U>
 __global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
@@ -507,6 +545,7 @@ void k_push(PushArgs a)
 	__shared__ int s_kcount;
 	__shared__ int s_w;
 	__shared__ double s_red[3][CPIC_BLOCK / 64];
+	__shared__ LvStage<T, IDS, PE> s_lv;
 
 	int gt = a.order[blockIdx.x];
 	int pl = find_plasma(a.pre_tile, a.P, gt);
@@ -534,6 +573,8 @@ void k_push(PushArgs a)
 	{
 		s_kcount = 0;
 		s_w = 0;
+		s_lv.n = 0;
+		s_lv.cut = LV_STAGE;
 	}
 	__syncthreads();
 
@@ -583,7 +624,8 @@ void k_push(PushArgs a)
 					leave[u] = push_one<T>(g, sE, E, sR, rho, tile, ox, oy, s, sk, set_r,
 							dtp, dtdx, dtdy, q[u], ndst[u], ke, mpx, mpy, bad, nan_);
 			}
-			// leavers: one atomic per wavefront
+			// leavers: staged in LDS (one LDS atomic per wavefront); once
+			// the stage is full, placed directly (one global atomic each)
 #pragma unroll
 			for(int u = 0; u < U; u++)
 			{
@@ -591,38 +633,56 @@ void k_push(PushArgs a)
 				if(lm)
 				{
 					int lcnt = __popcll(lm);
-					int lbase = 0;
+					int sb = 0;
 					if(lane == 0)
-						lbase = atomicAdd(&a.lv.count[a.par], lcnt);
-					lbase = __builtin_amdgcn_readfirstlane(lbase);
-					if(leave[u])
+						sb = atomicAdd(&s_lv.n, lcnt);
+					sb = __builtin_amdgcn_readfirstlane(sb);
+					if(sb + lcnt <= LV_STAGE)
 					{
-						int64_t ls = (int64_t) lbase + __popcll(lm & lt);
-						if(ls < a.lv.cap)
-							store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
-						else
+						if(leave[u])
 						{
-							// no room: stays in this bucket (misplaced);
-							// FP32 coordinates go back to this tile
-							leave[u] = false;
-							if(sizeof(T) == 4)
+							int k = sb + __popcll(lm & lt);
+							s_lv.x[k] = q[u].x;
+							s_lv.y[k] = q[u].y;
+							s_lv.ux[k] = q[u].ux;
+							s_lv.uy[k] = q[u].uy;
+							s_lv.dst[k] = ndst[u];
+							if(IDS)
+								s_lv.id[k] = q[u].id;
+							if(PE)
 							{
-								int dt_ = (ndst[u] - g.bucket0) / g.ns;
-								int dtx = (dt_ & (g.ntx - 1)) * g.tx - (ox + g.M);
-								int dty = (dt_ >> g.lgntx) * g.ty - (oy + g.M);
-								if(dtx > g.nx / 2) dtx -= g.nx;
-								if(dtx < -g.nx / 2) dtx += g.nx;
-								if(dty > g.ny / 2) dty -= g.ny;
-								if(dty < -g.ny / 2) dty += g.ny;
-								q[u].x = (T) ((float) q[u].x + (float) dtx);
-								q[u].y = (T) ((float) q[u].y + (float) dty);
+								s_lv.Ex[k] = q[u].Ex;
+								s_lv.Ey[k] = q[u].Ey;
 							}
-							atomicAdd(&a.ctr->misplaced, 1ull);
 						}
 					}
-					unsigned long long placed = __ballot(leave[u]);
-					if(lane == 0 && placed)
-						atomicAdd(&s_kcount, __popcll(placed));
+					else
+					{
+						// records [sb, LV_STAGE) stay unused; later
+						// wavefronts all start at or beyond LV_STAGE
+						if(lane == 0 && sb < LV_STAGE)
+							s_lv.cut = sb;
+						int lbase = 0;
+						if(lane == 0)
+							lbase = atomicAdd(&a.lv.count[a.par], lcnt);
+						lbase = __builtin_amdgcn_readfirstlane(lbase);
+						if(leave[u])
+						{
+							int64_t ls = (int64_t) lbase + __popcll(lm & lt);
+							if(ls < a.lv.cap)
+								store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
+							else
+							{
+								// no room: stays in this bucket (misplaced)
+								leave[u] = false;
+								back_to_tile<T>(g, ndst[u], ox, oy, q[u]);
+								atomicAdd(&a.ctr->misplaced, 1ull);
+							}
+						}
+						unsigned long long placed = __ballot(leave[u]);
+						if(lane == 0 && placed)
+							atomicAdd(&s_kcount, __popcll(placed));
+					}
 				}
 			}
 			// stayers: one LDS atomic per wavefront claims their slots in
@@ -643,6 +703,46 @@ void k_push(PushArgs a)
 			}
 		}
 
+		// the bucket's staged leavers: one global atomic claims their slots
+		__syncthreads();
+		if(tid == 0)
+		{
+			int nst = min(s_lv.n, s_lv.cut);
+			s_lv.n = nst;
+			s_lv.base = nst > 0 ? atomicAdd(&a.lv.count[a.par], nst) : 0;
+			if(nst > 0)
+			{
+				int64_t room = a.lv.cap - (int64_t) s_lv.base;
+				s_kcount += (int) (room <= 0 ? 0 : room < nst ? room : nst);
+			}
+		}
+		__syncthreads();
+		{
+			const int nst = s_lv.n;
+			const int64_t lb0 = s_lv.base;
+			for(int k = tid; k < nst; k += CPIC_BLOCK)
+			{
+				PState<T> lq;
+				lq.x = s_lv.x[k];
+				lq.y = s_lv.y[k];
+				lq.ux = s_lv.ux[k];
+				lq.uy = s_lv.uy[k];
+				lq.id = IDS ? s_lv.id[k] : 0;
+				lq.Ex = PE ? s_lv.Ex[k] : 0.0;
+				lq.Ey = PE ? s_lv.Ey[k] : 0.0;
+				int nd = s_lv.dst[k];
+				if(lb0 + k < a.lv.cap)
+					store_l<T>(a.lv, lb0 + k, lq, IDS, PE, nd, b);
+				else
+				{
+					// no room: stays in this bucket (misplaced)
+					back_to_tile<T>(g, nd, ox, oy, lq);
+					int w = atomicAdd(&s_w, 1);
+					store_p<T>(a.q, bstart + w, lq, IDS, PE);
+					atomicAdd(&a.ctr->misplaced, 1ull);
+				}
+			}
+		}
 		__syncthreads();
 		if(tid == 0)
 		{
@@ -651,6 +751,8 @@ void k_push(PushArgs a)
 			a.bk.res[b] = s_kcount;
 			a.bk.n2[b] = 0;
 			s_kcount = 0;
+			s_lv.n = 0;
+			s_lv.cut = LV_STAGE;
 		}
 		__syncthreads();
 	}
@@ -1637,7 +1739,14 @@ cpic_parts_init(cpic_dev *d)
 		(const void *) k_gather<double>, (const void *) k_gather<float>,
 	};
 	for(unsigned i = 0; i < sizeof(k) / sizeof(k[0]); i++)
-		HIPCHK(hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize,
-				CPIC_LDS_LIMIT));
+	{
+		// dynamic LDS up to the limit, less the kernel's static LDS
+		hipFuncAttributes fa;
+		HIPCHK(hipFuncGetAttributes(&fa, k[i]));
+		size_t lim = CPIC_LDS_LIMIT;
+		if(lim + fa.sharedSizeBytes > 160 * 1024)
+			lim = 160 * 1024 - fa.sharedSizeBytes;
+		HIPCHK(hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int) lim));
+	}
 	return 0;
 }
