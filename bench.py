@@ -165,6 +165,13 @@ def cpu_baseline(w, frac=8, steps=10):
     }
     if single:
         out["single_core"] = single
+    # the reference's own best single-node figure for this grid, as published
+    # (perf/strong-scaling-fixed/csv/time.csv:2): not re-measured here
+    out["published_reference"] = {
+        "value": 1e8 / 3.184, "unit": "particle-steps/s", "kind": "published, not re-measured",
+        "config": "2048^2, 1e8 particles, FP64, full step 3.184 s/iter",
+        "hardware": "MareNostrum 4, 1 node, 48 cores",
+        "source": "perf/strong-scaling-fixed/csv/time.csv:2"}
     return out
 
 

@@ -41,6 +41,8 @@
 template <typename T> struct V2;
 template <> struct V2<double> { typedef double2 t; };
 template <> struct V2<float> { typedef float2 t; };
+// the same pairs as clang vectors, which load through address-space pointers
+template <typename T> struct N2 { typedef T t __attribute__((ext_vector_type(2))); };
 
 // Arguments of the tile kernels.  Block b handles batch tile order[b], finds
 // the tile's plasma in pre_tile and takes the plasma's geometry and
@@ -263,16 +265,18 @@ static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>
 	// memory path even for LDS and wait for every outstanding HBM access.
 	if(c.lx >= 0 && c.lx < g.W - 1 && c.ly >= 0 && c.ly < g.H - 1)
 	{
-		const LDS_AS T *l = (const LDS_AS T *) sE;     // (x, y) pairs
+		typedef typename N2<T>::t P2;
+		const LDS_AS P2 *l = (const LDS_AS P2 *) sE;    // (x, y) pairs
 		int i = c.ly * g.W + c.lx;
-		e00.x = l[2 * i];
-		e00.y = l[2 * i + 1];
-		e01.x = l[2 * (i + g.W)];
-		e01.y = l[2 * (i + g.W) + 1];
-		e10.x = l[2 * (i + 1)];
-		e10.y = l[2 * (i + 1) + 1];
-		e11.x = l[2 * (i + g.W + 1)];
-		e11.y = l[2 * (i + g.W + 1) + 1];
+		P2 a00 = l[i], a01 = l[i + g.W], a10 = l[i + 1], a11 = l[i + g.W + 1];
+		e00.x = a00.x;
+		e00.y = a00.y;
+		e01.x = a01.x;
+		e01.y = a01.y;
+		e10.x = a10.x;
+		e10.y = a10.y;
+		e11.x = a11.x;
+		e11.y = a11.y;
 	}
 	else
 	{
@@ -493,11 +497,13 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 // to the leaver buffer with one atomic per wavefront.
 //
 // Waves per SIMD the register allocation of k_push must allow.  Measured at
-// 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (79 VGPRs with
-// U = 2), the FP64 kernel loses from any bound (its state is twice the size).
+// 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (78 VGPRs with
+// U = 2), the FP64 kernel loses from any bound (its state is twice the size);
+// bounding the FP32 U = 4 kernel (95 VGPRs) to 6 waves spills and loses 12 %.
 #ifndef PUSH_MIN_WAVES_F32
 #define PUSH_MIN_WAVES_F32 6
 #endif
+
 
 // Leavers of one bucket wait in LDS and are placed in the leaver buffer with
 // one global atomic per bucket: a counter shared by every workgroup, hit once
