@@ -391,8 +391,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the library runs on one GPU; multi-GPU runs are not implemented "
+              "(the path would shard by Y slabs of tiles with halo and leaver exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     import cpic_amd as ca

@@ -68,7 +68,8 @@ def test_harmonic_1200_steps(golden_dir):
 
 
 def test_iter0_half_step_by_hand(golden_dir):
-    """harm.r0x[1] = 1 + dt * u^{1/2}, u^{-1/2} = -(dt/2)(q/m)E (survey 8c)."""
+    """harm.r0x[1] = 1 + dt * u^{1/2}, u^{-1/2} = -(dt/2)(q/m)E: the iteration-0
+    half back-step, checked by hand against the reference's data."""
     o = harmonic_oracle(1024)
     o.step()                       # iter 0: velocity back half a step, no move
     p = o.particles(0)
