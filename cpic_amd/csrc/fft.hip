@@ -37,21 +37,58 @@ static __device__ __forceinline__ double2 cmul(double2 a, double2 w)
 	return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
 }
 
-// w = exp(sign * 2 pi i idx / N), 0 <= idx < N, from the half table
-// tw[k] = exp(+2 pi i k / N), k <= N/2 (the second half is -tw[idx - N/2]).
-static __device__ __forceinline__ double2 twiddle_at(const double2 *__restrict__ tw, int lgN,
-		int idx, double sign)
+#define LDS_AS __attribute__((address_space(3)))
+
+// Twiddles of one block, in LDS: w(i) = exp(+2 pi i i/N) = hi[i >> a] *
+// lo[i & (2^a - 1)], a = ceil(lg N / 2), filled from the half table tw[k] =
+// exp(+2 pi i k/N), k <= N/2 (second half -tw[k - N/2]).  Two LDS reads and
+// a complex product per twiddle instead of a global load (L2 latency in
+// every Stockham stage); the product is within ~2 ulp of the table value.
+#define TW_LDS 128     // 2^a and 2^(lg N - a) for N <= 8192
+struct TwLds {
+	double2 lo[TW_LDS];
+	double2 hi[TW_LDS];
+};
+struct TwRef {
+	const LDS_AS double *lo, *hi;   // (re, im) pairs
+	int a;
+};
+
+static_assert((1 << 7) <= TW_LDS, "twiddle factors for N = 8192");
+
+// Fills t (all threads of the block; the caller's next barrier publishes it).
+static __device__ __forceinline__ TwRef tw_fill(TwLds &t, const double2 *__restrict__ tw,
+		int lgN)
 {
-	int half = 1 << (lgN - 1);
-	double2 w;
-	if(idx < half)
-		w = tw[idx];
-	else
+	int a = (lgN + 1) >> 1, b = lgN - a, half = 1 << (lgN - 1);
+	for(int i = threadIdx.x; i < (1 << a); i += blockDim.x)
+		t.lo[i] = tw[i];
+	for(int i = threadIdx.x; i < (1 << b); i += blockDim.x)
 	{
-		w = tw[idx - half];
-		w.x = -w.x;
-		w.y = -w.y;
+		int idx = i << a;
+		double2 w;
+		if(idx < half)
+			w = tw[idx];
+		else
+		{
+			w = tw[idx - half];
+			w.x = -w.x;
+			w.y = -w.y;
+		}
+		t.hi[i] = w;
 	}
+	TwRef r;
+	r.lo = (const LDS_AS double *) t.lo;
+	r.hi = (const LDS_AS double *) t.hi;
+	r.a = a;
+	return r;
+}
+
+// w = exp(sign * 2 pi i idx / N), 0 <= idx < N
+static __device__ __forceinline__ double2 twiddle_at(const TwRef &t, int idx, double sign)
+{
+	int ih = 2 * (idx >> t.a), il = 2 * (idx & ((1 << t.a) - 1));
+	double2 w = cmul(make_double2(t.hi[ih], t.hi[ih + 1]), make_double2(t.lo[il], t.lo[il + 1]));
 	w.y *= sign;
 	return w;
 }
@@ -95,7 +132,7 @@ static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
 // both.  Needs (n/R)*nbat <= (FFT_MAXV/R) * FFT_THREADS.
 template <int R>
 static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
-		int nbat, int lgp, const double2 *__restrict__ tw, int lgN, double sign)
+		int nbat, int lgp, const TwRef &tw, int lgN, double sign)
 {
 	constexpr int LGR = R == 4 ? 2 : 1;
 	constexpr int MAXB = FFT_MAXV / R;
@@ -120,7 +157,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 #pragma unroll
 			for(int r = 1; r < R; r++)
 				if(k)
-					v[m][r] = cmul(v[m][r], twiddle_at(tw, lgN, (r * k) << tsh, sign));
+					v[m][r] = cmul(v[m][r], twiddle_at(tw, (r * k) << tsh, sign));
 			dft_small<R>(v[m], sign);
 			jo[m] = bat * stride + ((i - k) << LGR) + k;
 		}
@@ -141,7 +178,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 // order in and out (Stockham autosort: radix-4 stages, one radix-2 stage
 // when lgn is odd), w = exp(sign 2 pi i/len).  Starts and ends at a barrier.
 static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
-		const double2 *__restrict__ tw, int lgN, double sign)
+		const TwRef &tw, int lgN, double sign)
 {
 	int lgp = 0;
 	if(lgn & 1)
@@ -168,6 +205,8 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	int m = nx >> 1, lgm = g.lgnx - 1;
 	int stride = m + 1;
 	int y0 = (blockIdx.x - pre[p]) * R;
+	__shared__ TwLds s_tw;
+	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
 	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
 	{
@@ -177,7 +216,7 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_block(sh, stride, lgm, R, tw, lgN, -1.0);
+		fft_block(sh, stride, lgm, R, twr, lgN, -1.0);
 
 	// X[k] = E + W^k O, X[m-k] = conj(E - W^k O), W = exp(-2 pi i/nx)
 	int npair = (m >> 1) + 1;
@@ -204,9 +243,7 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 			double er = 0.5 * (zk.x + zm.x), ei = 0.5 * (zk.y - zm.y);
 			double dr = 0.5 * (zk.x - zm.x), di = 0.5 * (zk.y + zm.y);
 			double2 O = make_double2(di, -dr);
-			double2 w = tw[k << tsh];
-			w.y = -w.y;
-			double2 wo = cmul(O, w);
+			double2 wo = cmul(O, twiddle_at(twr, k << tsh, -1.0));
 			b[k] = make_double2(er + wo.x, ei + wo.y);
 			b[m - k] = make_double2(er - wo.x, -(ei - wo.y));
 		}
@@ -243,12 +280,14 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	long long iter = st->iter;
 	double *energy = (diag && iter >= 0) ? &diag[g.diag0 + iter % ring].field : (double *) NULL;
 	double escale = -0.5 * g.e0 * invN;
+	__shared__ TwLds s_tw;
+	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
 		sh[i] = ghat[(int64_t) k0 * ny + i];
 	__syncthreads();
 	if(lgny > 0)
-		fft_block(sh, ny, lgny, ncol, tw, lgN, -1.0);
+		fft_block(sh, ny, lgny, ncol, twr, lgN, -1.0);
 
 	double e = 0.0;
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
@@ -279,7 +318,7 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgny > 0)
-		fft_block(sh, ny, lgny, ncol, tw, lgN, +1.0);
+		fft_block(sh, ny, lgny, ncol, twr, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
 		ghat[(int64_t) k0 * ny + i] = sh[i];
@@ -301,6 +340,8 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	int m = nx >> 1, lgm = g.lgnx - 1;
 	int stride = m + 1;
 	int y0 = (blockIdx.x - pre[p]) * R;
+	__shared__ TwLds s_tw;
+	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
 	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
 	{
@@ -331,8 +372,8 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		else
 		{
 			double2 xk = b[k], xm = b[m - k];
-			double2 wk = tw[k << tsh];          // e^{+2 pi i k/nx} = (c, s)
-			double2 wm = tw[(m - k) << tsh];    // e^{+2 pi i (m-k)/nx}
+			double2 wk = twiddle_at(twr, k << tsh, 1.0);         // e^{+2 pi i k/nx}
+			double2 wm = twiddle_at(twr, (m - k) << tsh, 1.0);   // e^{+2 pi i (m-k)/nx}
 			// for k
 			double2 A = make_double2(xk.x + xm.x, xk.y - xm.y);
 			double2 D = make_double2(xk.x - xm.x, xk.y + xm.y);
@@ -347,7 +388,7 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_block(sh, stride, lgm, R, tw, lgN, +1.0);
+		fft_block(sh, stride, lgm, R, twr, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
 	{
@@ -483,13 +524,20 @@ cpic_fft_init(cpic_dev *d)
 	free(tw);
 	if(e1 != hipSuccess || e2 != hipSuccess)
 		return CPIC_EHIP;
-	// one row of 8192 points needs 64 KB + 16 B of LDS, a column 128 KB
-	HIPCHK(hipFuncSetAttribute((const void *) k_rows_r2c,
-			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
-	HIPCHK(hipFuncSetAttribute((const void *) k_rows_c2r,
-			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
-	HIPCHK(hipFuncSetAttribute((const void *) k_cols,
-			hipFuncAttributeMaxDynamicSharedMemorySize, CPIC_LDS_LIMIT));
+	// one row of 8192 points needs 64 KB + 16 B of LDS, a column 128 KB;
+	// the dynamic limit leaves room for each kernel's static LDS
+	const void *kern[] = {(const void *) k_rows_r2c, (const void *) k_rows_c2r,
+		(const void *) k_cols};
+	for(unsigned i = 0; i < sizeof(kern) / sizeof(kern[0]); i++)
+	{
+		hipFuncAttributes fa;
+		HIPCHK(hipFuncGetAttributes(&fa, kern[i]));
+		size_t lim = CPIC_LDS_LIMIT;
+		if(lim + fa.sharedSizeBytes > 160 * 1024)
+			lim = 160 * 1024 - fa.sharedSizeBytes;
+		HIPCHK(hipFuncSetAttribute(kern[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+				(int) lim));
+	}
 	return 0;
 }
 
@@ -526,8 +574,11 @@ cpic_fft_blocking(Geo *g)
 int
 cpic_fft_blocking_ok(const Geo *g)
 {
-	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= CPIC_LDS_LIMIT &&
-		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= CPIC_LDS_LIMIT &&
+	// dynamic LDS, leaving 16 KB for the kernels' static LDS (twiddle
+	// factors, reduction buffer)
+	const int64_t lim = 144 * 1024;
+	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= lim &&
+		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= lim &&
 		(int64_t) g->R * (g->nx / 2) <= FFT_MAX_VALUES &&
 		(int64_t) g->C * g->ny <= FFT_MAX_VALUES;
 }
