@@ -58,6 +58,10 @@ struct cpic_adj {
 	int64_t segcap;
 	double *hist;             // W per iteration
 	int64_t histcap;
+	double *wk;               // recompute state of the reverse sweep, 4n doubles
+	hipEvent_t ev[4];         // device work of the last forward [0,1] and backward [2,3]
+	int evok;                 // events created
+	int evrec;                // bit 0: forward pair recorded, bit 1: backward pair
 	double *part;             // NRED partial sums
 	int *derr;
 	int64_t nsteps;
@@ -78,10 +82,11 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 	i11 = i1y * g.nx + i1x;
 }
 
-// Deposits.  The particles are kept sorted by their initial cell, so the
-// ADJ_PPT*CPIC_BLOCK consecutive particles of a block lie in a small patch
-// of cells: each block accumulates its deposit in LDS over the bounding box
-// of its cells and flushes the non-zero nodes with one global atomic each.
+// Deposits.  The particles are kept sorted by their initial cell (cells taken
+// round-robin within each block's chunk), so the ADJ_PPT*CPIC_BLOCK
+// consecutive particles of a block lie in a small patch of cells: each block
+// accumulates its deposit in LDS over the bounding box of its cells and
+// flushes the non-zero nodes with one global atomic each.
 // A block whose box is too large or wraps around the grid deposits with
 // global atomics directly.
 #define ADJ_PPT 8
@@ -104,7 +109,15 @@ static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy
 		s4[3] = -1;
 	}
 	__syncthreads();
-	if(hix >= 0)
+	// wavefront reduction first: four LDS atomics per wavefront
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		lox = min(lox, __shfl_xor(lox, off));
+		hix = max(hix, __shfl_xor(hix, off));
+		loy = min(loy, __shfl_xor(loy, off));
+		hiy = max(hiy, __shfl_xor(hiy, off));
+	}
+	if((threadIdx.x & 63) == 0 && hix >= 0)
 	{
 		atomicMin(&s4[0], lox);
 		atomicMax(&s4[1], hix);
@@ -152,17 +165,19 @@ static __device__ __forceinline__ void dep1(const Geo &g, const Box &b, double *
 	double v00 = (dxw * dyw) * c, v01 = (dxw * ry) * c, v10 = (rx * dyw) * c, v11 = (rx * ry) * c;
 	if(b.lds)
 	{
-		atomicAdd(&sacc[box_node(b, ix, iy, 0, 0)], v00);
-		atomicAdd(&sacc[box_node(b, ix, iy, 0, 1)], v01);
-		atomicAdd(&sacc[box_node(b, ix, iy, 1, 0)], v10);
-		atomicAdd(&sacc[box_node(b, ix, iy, 1, 1)], v11);
+		LDS_AS double *l = (LDS_AS double *) sacc;
+		lds_add(&l[box_node(b, ix, iy, 0, 0)], v00);
+		lds_add(&l[box_node(b, ix, iy, 0, 1)], v01);
+		lds_add(&l[box_node(b, ix, iy, 1, 0)], v10);
+		lds_add(&l[box_node(b, ix, iy, 1, 1)], v11);
 	}
 	else
 	{
-		unsafeAtomicAdd(&rho[wrap_node(g, ix, iy)], v00);
-		unsafeAtomicAdd(&rho[wrap_node(g, ix, iy + 1)], v01);
-		unsafeAtomicAdd(&rho[wrap_node(g, ix + 1, iy)], v10);
-		unsafeAtomicAdd(&rho[wrap_node(g, ix + 1, iy + 1)], v11);
+		GLB_AS double *m = gptr<double>(rho);
+		glb_add(&m[wrap_node(g, ix, iy)], v00);
+		glb_add(&m[wrap_node(g, ix, iy + 1)], v01);
+		glb_add(&m[wrap_node(g, ix + 1, iy)], v10);
+		glb_add(&m[wrap_node(g, ix + 1, iy + 1)], v11);
 	}
 }
 
@@ -187,9 +202,9 @@ static __device__ void box_flush(const Geo &g, const Box &b, const double *sacc,
 		int64_t gnode = wrap_node(g, b.x0 + lx, b.y0 + ly);
 		for(int k = 0; k < ncomp; k++)
 		{
-			double v = sacc[i * ncomp + k];
+			double v = ((const LDS_AS double *) sacc)[i * ncomp + k];
 			if(v != 0.0)
-				unsafeAtomicAdd(&out[gnode * out_stride + k], v);
+				glb_add(&gptr<double>(out)[gnode * out_stride + k], v);
 		}
 	}
 }
@@ -422,15 +437,17 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		{
 			if(b.lds)
 			{
+				LDS_AS double *l = (LDS_AS double *) sacc;
 				int nd = box_node(b, ix[u], iy[u], ddx[c], ddy[c]);
-				atomicAdd(&sacc[2 * nd], w[c] * lex[u]);
-				atomicAdd(&sacc[2 * nd + 1], w[c] * ley[u]);
+				lds_add(&l[2 * nd], w[c] * lex[u]);
+				lds_add(&l[2 * nd + 1], w[c] * ley[u]);
 			}
 			else
 			{
+				GLB_AS double *m = gptr<double>(lE);
 				int64_t nd = wrap_node(g, ix[u] + ddx[c], iy[u] + ddy[c]);
-				unsafeAtomicAdd(&lE[nd].x, w[c] * lex[u]);
-				unsafeAtomicAdd(&lE[nd].y, w[c] * ley[u]);
+				glb_add(&m[2 * nd], w[c] * lex[u]);
+				glb_add(&m[2 * nd + 1], w[c] * ley[u]);
 			}
 		}
 	}
@@ -508,6 +525,10 @@ adj_free(cpic_adj *a)
 	hipFree(a->E); hipFree(a->lE);
 	hipFree(a->lx); hipFree(a->ly); hipFree(a->lux); hipFree(a->luy);
 	hipFree(a->ck); hipFree(a->seg); hipFree(a->hist); hipFree(a->part); hipFree(a->derr);
+	hipFree(a->wk);
+	if(a->evok)
+		for(int i = 0; i < 4; i++)
+			hipEventDestroy(a->ev[i]);
 	free(a->hsp);
 	free(a->perm);
 	free(a->hbuf);
@@ -586,6 +607,15 @@ cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
 		return CPIC_ENOMEM;
 	}
 	memcpy(a->hsp, species, sizeof(int32_t) * n);
+	for(int i = 0; i < 4; i++)
+		if(hipEventCreate(&a->ev[i]) != hipSuccess)
+		{
+			for(int j = 0; j < i; j++)
+				hipEventDestroy(a->ev[j]);
+			adj_free(a);
+			return CPIC_EHIP;
+		}
+	a->evok = 1;
 	*out = a;
 	return 0;
 }
@@ -727,6 +757,45 @@ sort_by_cell(cpic_adj *a, const double *x, const double *y)
 	for(int64_t i = 0; i < n; i++)
 		a->perm[start[cell[i]]++] = i;
 	free(start);
+	// Within each block's chunk (same cells, so the same LDS box), take the
+	// cells round-robin: the 64 consecutive particles of a wavefront then
+	// lie in different cells, and their deposit atomics spread over the LDS
+	// banks instead of piling onto the 4 nodes of one cell.
+	int64_t *tmp = (int64_t *) malloc(sizeof(int64_t) * 4 * (ADJ_CHUNK + 1));
+	if(!tmp)
+	{
+		free(cell);
+		return CPIC_ENOMEM;
+	}
+	int64_t *rs = tmp + (ADJ_CHUNK + 1), *pos = rs + (ADJ_CHUNK + 1), *act = pos + (ADJ_CHUNK + 1);
+	for(int64_t c0 = 0; c0 < n; c0 += ADJ_CHUNK)
+	{
+		int64_t c1 = c0 + ADJ_CHUNK < n ? c0 + ADJ_CHUNK : n, nr = 0, k = 0;
+		for(int64_t j = c0; j < c1; j++)
+			if(j == c0 || cell[a->perm[j]] != cell[a->perm[j - 1]])
+				rs[nr++] = j;
+		rs[nr] = c1;
+		for(int64_t q = 0; q < nr; q++)
+		{
+			act[q] = q;
+			pos[q] = rs[q];
+		}
+		// one particle of every cell that still has one, per round: O(chunk)
+		for(int64_t na = nr; na > 0; )
+		{
+			int64_t w = 0;
+			for(int64_t q = 0; q < na; q++)
+			{
+				int64_t r = act[q];
+				tmp[k++] = a->perm[pos[r]++];
+				if(pos[r] < rs[r + 1])
+					act[w++] = r;
+			}
+			na = w;
+		}
+		memcpy(a->perm + c0, tmp, sizeof(int64_t) * (size_t) (c1 - c0));
+	}
+	free(tmp);
 	free(cell);
 	return 0;
 }
@@ -790,6 +859,8 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	}
 	HIPCHK(hipMemsetAsync(a->derr, 0, sizeof(int), a->d->stream));
 	a->have = 0;
+	a->evrec = 0;             // a backward belongs to the forward before it
+	HIPCHK(hipEventRecord(a->ev[0], a->d->stream));
 	if((ret = deposit(a, a->x, a->y, a->rho[0])))
 		return ret;
 	for(int64_t it = 0; it < nsteps; it++)
@@ -801,6 +872,8 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 		if((ret = forward_iter(a, it, &cur, a->hist + it)))
 			return ret;
 	}
+	HIPCHK(hipEventRecord(a->ev[1], a->d->stream));
+	a->evrec |= 1;
 	double *h = (double *) malloc(sizeof(double) * (size_t) nsteps);
 	if(!h)
 		return CPIC_ENOMEM;
@@ -827,6 +900,24 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 }
 
 extern "C" int
+cpic_adj_device_ms(cpic_adj_t *a, double *forward_ms, double *backward_ms)
+{
+	if(!a || !a->evok)
+		return CPIC_EINVAL;
+	float f = 0.0f, b = 0.0f;
+	hipSetDevice(a->d->device);
+	if(a->evrec & 1)
+		HIPCHK(hipEventElapsedTime(&f, a->ev[0], a->ev[1]));
+	if(a->evrec & 2)
+		HIPCHK(hipEventElapsedTime(&b, a->ev[2], a->ev[3]));
+	if(forward_ms)
+		*forward_ms = (a->evrec & 1) ? (double) f : -1.0;
+	if(backward_ms)
+		*backward_ms = (a->evrec & 2) ? (double) b : -1.0;
+	return 0;
+}
+
+extern "C" int
 cpic_adj_state(cpic_adj_t *a, double *x, double *y, double *ux, double *uy)
 {
 	if(!a || !a->have)
@@ -849,14 +940,19 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	const int64_t nck = (T + K - 1) / K;
 	if((ret = grow(&a->seg, &a->segcap, 2 * n * K)))
 		return ret;
+	if(!a->wk && hipMalloc(&a->wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
+	{
+		a->wk = NULL;
+		return CPIC_ENOMEM;
+	}
+	a->evrec &= ~2;
+	HIPCHK(hipEventRecord(a->ev[2], a->d->stream));
 	if((ret = zero(a, a->lx, n)) || (ret = zero(a, a->ly, n)) || (ret = zero(a, a->lux, n)) ||
 			(ret = zero(a, a->luy, n)))
 		return ret;
 	// keep the forward's final state: the recompute runs on copies
 	double *fx = a->x, *fy = a->y, *fux = a->ux, *fuy = a->uy;
-	double *wk = NULL;
-	if(hipMalloc(&wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
-		return CPIC_ENOMEM;
+	double *wk = a->wk;
 	a->x = wk;
 	a->y = wk + n;
 	a->ux = wk + 2 * n;
@@ -909,12 +1005,13 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 				ret = CPIC_EHIP;
 		}
 	}
+	if(!ret && hipEventRecord(a->ev[3], a->d->stream) == hipSuccess)
+		a->evrec |= 2;
 	hipStreamSynchronize(a->d->stream);
 	a->x = fx;
 	a->y = fy;
 	a->ux = fux;
 	a->uy = fuy;
-	hipFree(wk);
 	if(ret)
 		return ret;
 	double *src[4] = {a->lx, a->ly, a->lux, a->luy};

@@ -237,6 +237,13 @@ void cpic_kt_end(cpic_dev *d, int slot);
 		} \
 	} while(0)
 
+// Pointer qualifiers for LDS and global memory.  Kernels spell them out
+// where a call site has both an LDS and a global path: through generic
+// pointers the compiler may merge the two into one flat instruction, which
+// goes through the vector-memory path even for LDS.
+#define LDS_AS __attribute__((address_space(3)))
+#define GLB_AS __attribute__((address_space(1)))
+
 // Dynamic LDS limit requested for kernels with large tiles or rows: the
 // 160 KB of a gfx950 workgroup minus room for static LDS.
 #define CPIC_LDS_LIMIT (160 * 1024 - 8 * 1024)

@@ -37,8 +37,6 @@ static __device__ __forceinline__ double2 cmul(double2 a, double2 w)
 	return make_double2(fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x));
 }
 
-#define LDS_AS __attribute__((address_space(3)))
-
 // Twiddles of one block, in LDS: w(i) = exp(+2 pi i i/N) = hi[i >> a] *
 // lo[i & (2^a - 1)], a = ceil(lg N / 2), filled from the half table tw[k] =
 // exp(+2 pi i k/N), k <= N/2 (second half -tw[k - N/2]).  Two LDS reads and

@@ -129,31 +129,6 @@ static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
 #define PUSH_NT_STORE 0
 #endif
 
-#define LDS_AS __attribute__((address_space(3)))
-#define GLB_AS __attribute__((address_space(1)))
-
-// A pointer into device memory as a global-address-space pointer.
-template <typename T>
-static __device__ __forceinline__ GLB_AS T *gptr(const void *p)
-{
-	return (GLB_AS T *) (T *) p;
-}
-
-// FP64 atomic adds with the address space spelled out, so the LDS and the
-// global paths of one call site are never merged into a generic (flat) op.
-static __device__ __forceinline__ void lds_add(LDS_AS double *p, double v)
-{
-	__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-static __device__ __forceinline__ void glb_add(GLB_AS double *p, double v)
-{
-	[[clang::atomic(ignore_denormal_mode)]]
-	{
-		__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	}
-}
-
 // Per-particle state in registers.
 template <typename T> struct PState {
 	T x, y, ux, uy;

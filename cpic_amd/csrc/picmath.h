@@ -6,6 +6,27 @@
 
 #include "dev.h"
 
+// A pointer into device memory as a global-address-space pointer.
+template <typename T>
+static __device__ __forceinline__ GLB_AS T *gptr(const void *p)
+{
+	return (GLB_AS T *) (T *) p;
+}
+
+// FP64 atomic adds into LDS and into global memory.
+static __device__ __forceinline__ void lds_add(LDS_AS double *p, double v)
+{
+	__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+static __device__ __forceinline__ void glb_add(GLB_AS double *p, double v)
+{
+	[[clang::atomic(ignore_denormal_mode)]]
+	{
+		__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
 // FP64 cell index and relative offset (src/interpolate.c:39-66).  The Y
 // middle term uses dx (the reference passes dx[X], src/interpolate.c:88).
 static __device__ __forceinline__ void cell64(const Geo &g, double x, double y,

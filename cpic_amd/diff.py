@@ -65,6 +65,12 @@ class AdjointRun:
         ca.check(ca.lib().cpic_adj_state(self._h, *(ca._dp(a) for a in out)))
         return out
 
+    def device_ms(self):
+        """Device time of the last forward and backward in ms (-1 if not run)."""
+        f, b = C.c_double(), C.c_double()
+        ca.check(ca.lib().cpic_adj_device_ms(self._h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
 
 class FieldEnergyLoss(torch.autograd.Function):
     """loss = sum of field energies over nsteps iterations from (r0, u0)."""

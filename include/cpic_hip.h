@@ -259,6 +259,13 @@ int cpic_adj_backward(cpic_adj_t *adj, double dloss, double *gx, double *gy,
 /* State after the last forward (host arrays of n; any may be NULL). */
 int cpic_adj_state(cpic_adj_t *adj, double *x, double *y, double *ux, double *uy);
 
+/* Device time of the last forward and of the backward that followed it, in
+ * ms (HIP events on the library's stream from the end of the uploads to the
+ * start of the downloads, so host staging and PCIe transfers are excluded);
+ * -1 for one that has not run.  No counterpart in the reference, which has
+ * no differentiable run. */
+int cpic_adj_device_ms(cpic_adj_t *adj, double *forward_ms, double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,7 +89,11 @@ def test_adjoint_gradient_matches_torch_reference_64():
     assert max(errs.values()) < 1e-7, errs
     # the checkpoint interval changes only the recompute schedule
     run.forward(*c["st"], 200, 1)
+    f_ms, b_ms = run.device_ms()
+    assert f_ms > 0 and b_ms == -1.0          # this forward's backward has not run
     g1 = run.backward(1.0)
+    f2, b_ms = run.device_ms()
+    assert f2 == f_ms and b_ms > 0
     run.forward(*c["st"], 200, 200)
     g200 = run.backward(1.0)
     for a, b in zip(g1, g200):
