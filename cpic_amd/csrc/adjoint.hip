@@ -89,9 +89,15 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 // flushes the non-zero nodes with one global atomic each.
 // A block whose box is too large or wraps around the grid deposits with
 // global atomics directly.
-#define ADJ_PPT 8
+// 4 particles per thread and 1,024 per block: at 8 the kernels hold 100+
+// VGPRs (4 waves per SIMD) and the 256^2 adjoint workload runs 10 % slower
+#ifndef ADJ_PPT
+#define ADJ_PPT 4
+#endif
 #define ADJ_CHUNK (ADJ_PPT * CPIC_BLOCK)
-#define ADJ_BOX_NODES 3072          // 48 KB of double2, 24 KB of double
+#ifndef ADJ_BOX_NODES
+#define ADJ_BOX_NODES 3072          // 24 KB of double
+#endif
 
 struct Box {
 	int x0, y0, w, h;
@@ -99,7 +105,8 @@ struct Box {
 };
 
 // Bounding box (plus the CIC +1 node) of the block's cells [lo, hi].
-static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy, int *s4)
+static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy, int *s4,
+		int cap)
 {
 	if(threadIdx.x == 0)
 	{
@@ -131,7 +138,7 @@ static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy
 	b.w = s4[1] - s4[0] + 2;
 	b.h = s4[3] - s4[2] + 2;
 	b.lds = s4[1] >= 0 && s4[1] - s4[0] <= g.nx / 2 && s4[3] - s4[2] <= g.ny / 2 &&
-		b.w * b.h <= ADJ_BOX_NODES;
+		b.w * b.h <= cap;
 	__syncthreads();
 	return b;
 }
@@ -234,7 +241,7 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 			hiy = max(hiy, iy[u]);
 		}
 	}
-	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	Box b = block_box(g, lox, hix, loy, hiy, s4, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
@@ -314,7 +321,7 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		atomicOr(derr, CPIC_DERR_UMAX);
 	if(nan_)
 		atomicOr(derr, CPIC_DERR_NAN);
-	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	Box b = block_box(g, lox, hix, loy, hiy, s4, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
@@ -367,6 +374,15 @@ void k_adj_sum(const double *__restrict__ part, double scale, double *__restrict
 
 // Reverse of move + Boris + gather for one iteration; accumulates the
 // gather-weight derivative into (lx, ly) and deposits lE_p into lE.
+// REV_PPT particles per thread: the kernel keeps six values per particle
+// across the box barriers, and 8 per thread would hold it to 3 waves per
+// SIMD.  A fraction of a round-robin chunk still spans its cells.
+#ifndef REV_PPT
+#define REV_PPT 4
+#endif
+#define REV_CHUNK (REV_PPT * CPIC_BLOCK)
+static_assert(ADJ_CHUNK % REV_CHUNK == 0, "reverse blocks split the sort's chunks");
+#define REV_BOX_NODES 1536          // 2 components: 24 KB
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		const int32_t *__restrict__ sp, const double *__restrict__ x,
@@ -374,14 +390,14 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double *__restrict__ ly, double *__restrict__ lux, double *__restrict__ luy,
 		double2 *__restrict__ lE)
 {
-	__shared__ double sacc[2 * ADJ_BOX_NODES];
+	__shared__ double sacc[2 * REV_BOX_NODES];
 	__shared__ int s4[4];
-	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
-	int ix[ADJ_PPT], iy[ADJ_PPT];
-	double rx[ADJ_PPT], ry[ADJ_PPT], lex[ADJ_PPT], ley[ADJ_PPT];
+	int64_t p0 = (int64_t) blockIdx.x * REV_CHUNK;
+	int ix[REV_PPT], iy[REV_PPT];
+	double rx[REV_PPT], ry[REV_PPT], lex[REV_PPT], ley[REV_PPT];
 	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
 #pragma unroll
-	for(int u = 0; u < ADJ_PPT; u++)
+	for(int u = 0; u < REV_PPT; u++)
 	{
 		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
 		ix[u] = -1;
@@ -422,10 +438,10 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		loy = min(loy, iy[u]);
 		hiy = max(hiy, iy[u]);
 	}
-	Box b = block_box(g, lox, hix, loy, hiy, s4);
+	Box b = block_box(g, lox, hix, loy, hiy, s4, REV_BOX_NODES);
 	box_zero(b, sacc, 2);
 #pragma unroll
-	for(int u = 0; u < ADJ_PPT; u++)
+	for(int u = 0; u < REV_PPT; u++)
 	{
 		if(ix[u] < 0)
 			continue;
@@ -986,7 +1002,8 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			if((ret = deposit(a, sx, sy, a->rho[0])) || (ret = fields(a, a->rho[0], NULL)) ||
 					(ret = zero(a, (double *) a->lE, 2 * nodes)))
 				break;
-			hipLaunchKernelGGL(k_adj_rev_a, dim3(chunks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+			hipLaunchKernelGGL(k_adj_rev_a, dim3((int) ((n + REV_CHUNK - 1) / REV_CHUNK)),
+					dim3(CPIC_BLOCK), 0, a->d->stream,
 					a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
 					(const double2 *) a->E, a->lx, a->ly, a->lux, a->luy, a->lE);
 			hipLaunchKernelGGL(k_adj_field, dim3(pblocks(nodes)), dim3(CPIC_BLOCK), 0,
