@@ -47,7 +47,7 @@ struct cpic_adj {
 	int32_t *sp;              // species of each particle (device, in sorted order)
 	int32_t *hsp;             // species of each particle (host, caller's order)
 	int64_t *perm;            // sorted position i holds the caller's particle perm[i]
-	double *hbuf;             // host staging, 4n doubles
+	double *hbuf;             // host staging, 4n doubles (pinned)
 	double *x, *y, *ux, *uy;  // state
 	double *rho[2], *phi, *psi, *lrho;
 	double2 *E, *lE;
@@ -547,7 +547,8 @@ adj_free(cpic_adj *a)
 			hipEventDestroy(a->ev[i]);
 	free(a->hsp);
 	free(a->perm);
-	free(a->hbuf);
+	if(a->hbuf)
+		hipHostFree(a->hbuf);
 	if(a->d)
 		cpic_dev_destroy(a->d);
 	free(a);
@@ -616,7 +617,9 @@ cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
 	}
 	a->hsp = (int32_t *) malloc(sizeof(int32_t) * (size_t) n);
 	a->perm = (int64_t *) malloc(sizeof(int64_t) * (size_t) n);
-	a->hbuf = (double *) malloc(sizeof(double) * 4 * (size_t) n);
+	if(hipHostMalloc((void **) &a->hbuf, sizeof(double) * 4 * (size_t) n, hipHostMallocDefault)
+			!= hipSuccess)
+		a->hbuf = NULL;
 	if(!a->hsp || !a->perm || !a->hbuf)
 	{
 		adj_free(a);
