@@ -136,7 +136,7 @@ def run_both(o, d, nsteps):
     return np.array(W)
 
 
-@pytest.mark.parametrize("tile", [(16, 16), (8, 8), (4, 16), (64, 64)])
+@pytest.mark.parametrize("tile", [(16, 16), (8, 8), (4, 16), (16, 8), (32, 16), (64, 64)])
 def test_20_steps_fp64_rel_l2(tile):
     o, d = c1(seed=4, tile=tile)
     W = run_both(o, d, 20)
@@ -343,11 +343,12 @@ def test_c2_scale_fp64_matches_oracle():
     print(f"500 steps {time.time() - t0:.1f} s", flush=True)
 
 
-@pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-5)])
-def test_four_particles_per_thread(precision, tol):
+@pytest.mark.parametrize("precision,tol,tile", [(64, 1e-12, (8, 8)), (32, 1e-5, (8, 8)),
+                                                (32, 1e-5, (32, 16)), (32, 1e-5, (16, 8))])
+def test_four_particles_per_thread(precision, tol, tile):
     """The push kernel's ppt = 4 instantiation (autotuner choice) gives the
-    same results as the default."""
-    o, d = c1(seed=12, precision=precision, ppt=4, tile=(8, 8))
+    same results as the default, at the tuned tile shapes too."""
+    o, d = c1(seed=12, precision=precision, ppt=4, tile=tile)
     W = run_both(o, d, 20)
     d.sync()
     assert rel_l2(d.field("rho"), o.field("rho")) < tol
