@@ -124,11 +124,6 @@ static __device__ __forceinline__ int64_t bucket_slot(const Buckets &bk, int b,
 	return j < n1 ? s + j : s + (bk.cap[b] - bk.res[b]) + (j - n1);
 }
 
-// 1: particle-state stores bypass the caches' retention (nontemporal)
-#ifndef PUSH_NT_STORE
-#define PUSH_NT_STORE 0
-#endif
-
 // Per-particle state in registers.
 template <typename T> struct PState {
 	T x, y, ux, uy;
@@ -176,18 +171,10 @@ static __device__ __forceinline__ void store_p(const Parts &p, int64_t s, const 
 		bool ids, bool pe)
 {
 	GLB_AS T *r = gptr<T>(p.r) + pslot(s);
-#if PUSH_NT_STORE
-	// streaming stores: the state is read back only in the next step
-	__builtin_nontemporal_store(q.x, &r[0]);
-	__builtin_nontemporal_store(q.y, &r[PGROUP]);
-	__builtin_nontemporal_store(q.ux, &r[2 * PGROUP]);
-	__builtin_nontemporal_store(q.uy, &r[3 * PGROUP]);
-#else
 	r[0] = q.x;
 	r[PGROUP] = q.y;
 	r[2 * PGROUP] = q.ux;
 	r[3 * PGROUP] = q.uy;
-#endif
 	if(ids)
 		p.id[s] = q.id;
 	if(pe)
@@ -457,12 +444,6 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 	}
 }
 
-// 1: k_push issues the next chunk's loads before pushing the current one
-#ifndef PUSH_PREFETCH
-#define PUSH_PREFETCH 0
-#endif
-
-
 // Fused gather + push + wrap + deposit + tile compaction.  Each thread
 // handles U particles per chunk (particle base + u*256 + tid), all loads of
 // a chunk issued before any of them is used, so a workgroup keeps U*256
@@ -572,30 +553,12 @@ void k_push(PushArgs a)
 		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
 		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 
-#if PUSH_PREFETCH
-		PState<T> nq[U];
-		bool nact[U];
-		load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, 0, tid, nq, nact);
-#endif
 		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
 		{
 			PState<T> q[U];
 			bool act[U], leave[U];
 			int ndst[U];
-#if PUSH_PREFETCH
-			// the next chunk's loads are in flight while this one is pushed;
-			// its slots lie beyond everything this chunk writes
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				q[u] = nq[u];
-				act[u] = nact[u];
-			}
-			load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base + U * CPIC_BLOCK, tid, nq,
-					nact);
-#else
 			load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
-#endif
 #pragma unroll
 			for(int u = 0; u < U; u++)
 			{
