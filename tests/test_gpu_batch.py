@@ -206,3 +206,27 @@ def test_batch_upload_replaces_one_plasma():
             assert np.array_equal(a[k], before[p][k]), (p, k)
     got = b.particles(1, plasma=2)
     assert np.array_equal(np.sort(got["x"]), np.sort(x[:100]))
+
+
+def test_batch_with_an_empty_plasma():
+    """A plasma with no particles at all between two populated ones (graph
+    replay on): the populated ones match the oracle at 1e-12, the empty one
+    has zero fields, no particles and zero field energies."""
+    cfgs = plasmas()[:3]
+    c = cfgs[1]
+    empty = (np.empty(0, np.int64),) + (np.empty(0),) * 4
+    cfgs[1] = dict(c, parts=[empty for _ in c["q"]])
+    b = make_batch(cfgs)
+    b.pre_step()
+    b.run(0, 20)
+    b.sync()
+    for p, c in enumerate(cfgs):
+        o = make_oracle(c)
+        W = oracle_run(o, 20)
+        check_plasma(b, p, c, o, W, 1e-12)
+    for s in range(len(cfgs[1]["q"])):
+        assert b.count(s, plasma=1) == 0
+        assert b.particles(s, plasma=1)["x"].size == 0
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert not np.any(b.field(f, plasma=1))
+    assert not np.any(b.energy(plasma=1)["field"][:20])

@@ -398,3 +398,25 @@ def test_fast_streams_overflow_leaver_stage_and_buffer(precision, tol):
         a = o.particles(s)
         assert np.array_equal(d.cell_counts(s), np.bincount(
             o.cells(a["x"], a["y"]), minlength=64 * 64).reshape(64, 64))
+
+
+def test_empty_single_and_ragged_species():
+    """Species with 999 particles (no whole wavefront or block), one with a
+    single particle and one with none: 20 steps match the oracle at 1e-12;
+    the empty species downloads empty."""
+    L = (64.0, 64.0)
+    parts = two_stream(999, L, 0.1, 0.01, 31)
+    one = (np.array([7], np.int64), np.array([10.3]), np.array([20.7]), np.array([0.05]),
+           np.array([-0.02]))
+    empty = (np.empty(0, np.int64),) + (np.empty(0),) * 4
+    parts = [parts[0], parts[1], one, empty]
+    m = 2 * 999 / (L[0] * L[1])
+    o, d = make_pair(64, 64, L, 0.1, [-1.0, -1.0, 1.0, 1.0], [m, m, 50.0, 50.0], parts,
+                     tile=(16, 16))
+    W = run_both(o, d, 20)
+    d.sync()
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert rel_l2(d.field(f), o.field(f)) < 1e-12, f
+    assert_same_particles(o, d, 3, exact=False, tol=1e-12)
+    assert d.count(3) == 0 and d.particles(3)["x"].size == 0
+    assert rel_l2(d.energy()["field"], W) < 1e-12
