@@ -420,3 +420,41 @@ def test_empty_single_and_ragged_species():
     assert_same_particles(o, d, 3, exact=False, tol=1e-12)
     assert d.count(3) == 0 and d.particles(3)["x"].size == 0
     assert rel_l2(d.energy()["field"], W) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(8192, 8192), (8192, 2), (2, 8192)])
+def test_solve_at_maximum_and_extreme_grids(nx, ny):
+    """The MFT solve at the largest grid the library accepts (8192^2: one
+    row per row-pass block, one 128 KB column per column-pass block, the
+    LDS twiddle factors at N = 8192) and at the thinnest ones: for rho a sum
+    of two Fourier modes, phi = G(k) rho mode by mode (closed form).  The
+    solve is ill-conditioned at these sizes: the rounding of the sampled
+    modes (~eps) reaches the lowest modes, where |G| is largest (about
+    (N / 2 pi)^2 = 1.7e6 at N = 8192), so the bound is 64 eps max|G| sum|a|
+    (numpy's pocketfft misses the closed form by the same 1e-11 at 8192 x 2;
+    a wrong G, mode or normalisation misses it by O(|phi|))."""
+    import math
+    d = ca.Device(nx, ny, (float(nx), float(ny)), 0.1, [-1.0], [1.0])
+    x = np.arange(nx)[None, :]
+    y = np.arange(ny)[:, None]
+    modes = [(min(3, nx // 2), min(5, ny // 2), 1.0), (nx // 2, 1 if ny > 2 else 0, 0.25)]
+    rho = np.zeros((ny, nx))
+    phi = np.zeros((ny, nx))
+    n_max = max(nx, ny)
+    gmax = 1.0 / (2.0 - 2.0 * math.cos(2 * math.pi / n_max))
+    tol = 64 * np.finfo(float).eps * gmax * sum(abs(a) for _, _, a in modes)
+    for kx, ky, a in modes:
+        if kx == 0 and ky == 0:
+            continue
+        g = np.cos(2 * np.pi * (kx * x / nx + ky * y / ny)) * a
+        G = 1.0 / (2.0 * (math.cos(2 * math.pi * kx / nx) + math.cos(2 * math.pi * ky / ny)) - 4.0)
+        rho += g
+        phi += G * g
+    del x, y, g
+    d.set_field("rho", rho)
+    d.solve_xy()
+    got = d.field("phi")
+    err = np.max(np.abs(got - phi))
+    assert err < tol, (nx, ny, err, tol)
+    assert np.max(np.abs(phi)) > 1e3 * tol          # the check is not vacuous
+    d.close()
