@@ -178,10 +178,10 @@ def cpu_baseline(w, frac=8, steps=10):
     return out
 
 
-def traffic_from_profiles(precision):
+def traffic_from_profiles(precision, name=None):
     """HBM bytes per push launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if present."""
-    path = os.path.join(ROOT, "profiles", f"pmc_push_fp{precision}_2048.txt")
+    path = os.path.join(ROOT, "profiles", name or f"pmc_push_fp{precision}_2048.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -294,7 +294,8 @@ def run_ensemble(args, ca):
                    "ppt": ppt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                     "traffic": traffic_from_profiles(32, "pmc_push_fp32_ensemble_256plasmas.txt"),
                      "kernel": "push_gather_deposit", "alg_bytes_per_launch": alg,
                      "avg_launch_ms": avg_push * 1e3},
         "kernels_ms": {k: ms / max(n, 1) for k, (n, ms) in kt.items()},
@@ -405,14 +406,14 @@ def main():
         sys.exit(2)
 
     import cpic_amd as ca
+    if args.calibrate_copy:
+        copy_calibration()
     if args.workload == "ensemble":
         run_ensemble(args, ca)
         return
     if args.workload == "adjoint":
         run_adjoint(args)
         return
-    if args.calibrate_copy:
-        copy_calibration()
     w = workload(args.grid, args.ppc)
     ntot = sum(w["n"])
     ng = args.grid * args.grid
