@@ -91,11 +91,41 @@ static __device__ __forceinline__ double2 twiddle_at(const TwRef &t, int idx, do
 	return w;
 }
 
-// In-register R-point DFT (R = 2 or 4), exp(sign 2 pi i / R) roots.
+// In-register R-point DFT (R = 2, 4 or 8), exp(sign 2 pi i / R) roots.
+template <int R>
+static __device__ __forceinline__ void dft_small(double2 *v, double sign);
+
+// 8 points: two 4-point DFTs of the even and odd points, combined with
+// w^k, w = exp(sign 2 pi i / 8): w^1 = (c, sign c), w^2 = (0, sign),
+// w^3 = (-c, sign c), c = sqrt(1/2).
+static __device__ __forceinline__ void dft8(double2 *v, double sign)
+{
+	double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+	dft_small<4>(e, sign);
+	dft_small<4>(o, sign);
+	const double c = 0.70710678118654752440;
+	double2 t[4];
+	t[0] = o[0];
+	// w^1 o1 = c (o1.x - sign o1.y, o1.y + sign o1.x)
+	t[1] = make_double2(c * (o[1].x - sign * o[1].y), c * (o[1].y + sign * o[1].x));
+	// w^2 o2 = sign i o2 = (-sign o2.y, sign o2.x)
+	t[2] = make_double2(-sign * o[2].y, sign * o[2].x);
+	// w^3 o3 = c (-o3.x - sign o3.y, -o3.y + sign o3.x)
+	t[3] = make_double2(c * (-o[3].x - sign * o[3].y), c * (-o[3].y + sign * o[3].x));
+#pragma unroll
+	for(int k = 0; k < 4; k++)
+	{
+		v[k] = make_double2(e[k].x + t[k].x, e[k].y + t[k].y);
+		v[k + 4] = make_double2(e[k].x - t[k].x, e[k].y - t[k].y);
+	}
+}
+
 template <int R>
 static __device__ __forceinline__ void dft_small(double2 *v, double sign)
 {
-	if(R == 2)
+	if(R == 8)
+		dft8(v, sign);
+	else if(R == 2)
 	{
 		double2 a = v[0], b = v[1];
 		v[0] = make_double2(a.x + b.x, a.y + b.y);
@@ -132,7 +162,7 @@ template <int R>
 static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
 		int nbat, int lgp, const TwRef &tw, int lgN, double sign)
 {
-	constexpr int LGR = R == 4 ? 2 : 1;
+	constexpr int LGR = R == 8 ? 3 : R == 4 ? 2 : 1;
 	constexpr int MAXB = FFT_MAXV / R;
 	const int lgnb = lgn - LGR, nb = 1 << lgnb, p = 1 << lgp;
 	const int total = nb * nbat;
@@ -173,19 +203,26 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 }
 
 // FFT of nbat transforms of length n = 2^lgn at buf + b*stride, natural
-// order in and out (Stockham autosort: radix-4 stages, one radix-2 stage
-// when lgn is odd), w = exp(sign 2 pi i/len).  Starts and ends at a barrier.
+// order in and out (Stockham autosort: radix-8 stages after one radix-2 or
+// radix-4 stage for lg n mod 3), w = exp(sign 2 pi i/len).  Starts and ends
+// at a barrier.
 static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
 		const TwRef &tw, int lgN, double sign)
 {
+	// radix-8 stages, after one radix-2 or radix-4 stage for lg n mod 3
 	int lgp = 0;
-	if(lgn & 1)
+	if(lgn % 3 == 1)
 	{
 		stockham_stage<2>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 		lgp = 1;
 	}
-	for(; lgp < lgn; lgp += 2)
+	else if(lgn % 3 == 2)
+	{
 		stockham_stage<4>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		lgp = 2;
+	}
+	for(; lgp < lgn; lgp += 3)
+		stockham_stage<8>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 }
 
 // Pass 1: R rows per block; row y real of length nx = 2m.
