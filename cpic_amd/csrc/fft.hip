@@ -301,7 +301,7 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		int ring, const StepState *__restrict__ st)
 {
 	extern __shared__ double2 sh[];
-	__shared__ double red[FFT_THREADS];
+	__shared__ double red[FFT_THREADS / 64];
 	int p = find_plasma(pre, P, blockIdx.x);
 	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, C = g.C;
@@ -340,16 +340,19 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	if(energy)
 	{
-		red[threadIdx.x] = e;
+		// wavefront sums, then one barrier
+		for(int off = 32; off > 0; off >>= 1)
+			e += __shfl_down(e, off);
+		if((threadIdx.x & 63) == 0)
+			red[threadIdx.x >> 6] = e;
 		__syncthreads();
-		for(int s = FFT_THREADS / 2; s > 0; s >>= 1)
-		{
-			if((int) threadIdx.x < s)
-				red[threadIdx.x] += red[threadIdx.x + s];
-			__syncthreads();
-		}
 		if(threadIdx.x == 0)
-			unsafeAtomicAdd(energy, red[0] * escale);
+		{
+			double t = 0.0;
+			for(int w = 0; w < FFT_THREADS / 64; w++)
+				t += red[w];
+			unsafeAtomicAdd(energy, t * escale);
+		}
 	}
 	__syncthreads();
 	if(lgny > 0)
@@ -610,7 +613,7 @@ int
 cpic_fft_blocking_ok(const Geo *g)
 {
 	// dynamic LDS, leaving 16 KB for the kernels' static LDS (twiddle
-	// factors, reduction buffer)
+	// factors, reduction partials)
 	const int64_t lim = 144 * 1024;
 	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= lim &&
 		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= lim &&
