@@ -386,10 +386,12 @@ parse_settings(lex_t *L, cfg_node_t *group, int close)
 			if(*L->p != '"' || parse_string(L, &file))
 				return fail(L, "@include needs a file name");
 			if(file[0] == '/')
-				snprintf(full, sizeof(full), "%s", file);
+				r = snprintf(full, sizeof(full), "%s", file);
 			else
-				snprintf(full, sizeof(full), "%s/%s", L->dir, file);
+				r = snprintf(full, sizeof(full), "%s/%s", L->dir, file);
 			free(file);
+			if(r < 0 || r >= (int) sizeof(full))
+				return fail(L, "@include path too long");
 			r = include_file(L, group, full);
 			if(r)
 				return r;
@@ -466,6 +468,11 @@ include_file(lex_t *L, cfg_node_t *group, const char *path)
 	sub.c = L->c;
 	sub.depth = L->depth + 1;
 	tmp = strdup(path);
+	if(!tmp)
+	{
+		free(text);
+		return fail(L, "out of memory");
+	}
 	snprintf(sub.dir, sizeof(sub.dir), "%s", dirname(tmp));
 	free(tmp);
 	r = parse_settings(&sub, group, 0);

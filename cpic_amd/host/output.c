@@ -96,11 +96,9 @@ output_init(sim_t *sim, output_t *out)
 	}
 	if(make_dir(out->path))
 		goto fail;
-	snprintf(p, sizeof(p), "%s/xdmf", out->path);
-	if(make_dir(p))
+	if(snprintf(p, sizeof(p), "%s/xdmf", out->path) >= (int) sizeof(p) || make_dir(p))
 		goto fail;
-	snprintf(p, sizeof(p), "%s/bin", out->path);
-	if(make_dir(p))
+	if(snprintf(p, sizeof(p), "%s/bin", out->path) >= (int) sizeof(p) || make_dir(p))
 		goto fail;
 	out->enabled = 1;
 	return 0;
