@@ -458,3 +458,35 @@ def test_solve_at_maximum_and_extreme_grids(nx, ny):
     assert err < tol, (nx, ny, err, tol)
     assert np.max(np.abs(phi)) > 1e3 * tol          # the check is not vacuous
     d.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_long_run_conservation_with_fast_beams(precision):
+    """3,000 steps of fast beams (0.5 cell per step) on 8x8 tiles, replayed
+    from captured graphs with a re-sort after step 1,500: the particle count
+    of each species, the total charge sum(rho) = N and finiteness hold at
+    every check; leavers cross tiles every step."""
+    n, L = 262144, (128.0, 128.0)
+    m = 2 * n / (L[0] * L[1])
+    parts = two_stream(n, L, 5.0, 0.5, seed=41)
+    d = ca.Device(128, 128, L, 0.1, [-1.0, -1.0], [m, m], precision=precision, tile=(8, 8))
+    for s, (ids, x, y, ux, uy) in enumerate(parts):
+        d.upload(s, ids, x, y, ux, uy)
+    d.pre_step()
+    it = 0
+    tol = 1e-9 if precision == 64 else 1e-5
+    for block in range(6):
+        d.run(it, 500)
+        it += 500
+        d.sync()
+        assert d.count(0) == n and d.count(1) == n, it
+        rho = d.field("rho")
+        assert np.all(np.isfinite(rho)), it
+        assert abs(rho.sum() - 2 * n) < tol * 2 * n, (it, rho.sum())   # -q/e0 = 1 each
+        if block == 2:
+            d.rebuild()
+    st = d.stats()
+    assert st["leavers"] > 100 * n // 8
+    W = d.energy()["field"][:it]
+    assert np.all(np.isfinite(W))
+    d.close()
