@@ -570,7 +570,42 @@ commit(cpic_dev *d)
 	if((ret = cpic_parts_rebuild(d)))
 		return ret;
 	HIPCHK(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
+	d->misplaced_seen = 0;
 	return 0;
+}
+
+// A leaver buffer that overflowed left particles in buckets whose tile they
+// had left: still correct, but gathered and deposited through the global
+// path, and in FP32 with coordinates drifting away from their tile; and they
+// try to leave again every step, so the demand snowballs.  At a sync point
+// (between stages, no leaver in flight), when that has happened since the
+// last check, the buffer grows to 1.25x the last step's demand (at least
+// double, at most every particle) and the particles are re-sorted.  The
+// counters stay cumulative.
+static int
+heal_overflow(cpic_dev *d)
+{
+	Counters c;
+	int ret;
+	HIPCHK(hipMemcpy(&c, d->ctr, sizeof(c), hipMemcpyDeviceToHost));
+	if(c.misplaced <= d->misplaced_seen)
+		return 0;
+	d->misplaced_seen = c.misplaced;
+	int64_t tot = 0;
+	for(int p = 0; p < d->P; p++)
+		for(int s = 0; s < d->hgeo[p].ns; s++)
+			tot += species_count(d, p, s);
+	int32_t demand = 0;
+	HIPCHK(hipMemcpy(&demand, d->lv.count + (1 - d->step_parity), sizeof(int32_t),
+			hipMemcpyDeviceToHost));
+	int64_t want = 2 * d->lv.cap;
+	if(want < (int64_t) demand + demand / 4)
+		want = (int64_t) demand + demand / 4;
+	if(want > tot + 16384)
+		want = tot + 16384;
+	if(want > d->lv.cap && (ret = leavers_alloc(d, want)))
+		return ret;
+	return cpic_parts_rebuild(d);
 }
 
 static int
@@ -593,7 +628,13 @@ cpic_dev_sync(cpic_dev_t *d)
 		return CPIC_EINVAL;
 	set_device(d);
 	HIPCHK(hipStreamSynchronize(d->stream));
-	return derr_check(d);
+	int ret = derr_check(d);
+	if(ret || d->pending)
+		return ret;
+	if((ret = heal_overflow(d)))
+		return ret;
+	HIPCHK(hipStreamSynchronize(d->stream));
+	return 0;
 }
 
 extern "C" int
@@ -665,6 +706,7 @@ cpic_dev_rebuild(cpic_dev_t *d)
 	if((ret = cpic_parts_rebuild(d)))
 		return ret;
 	HIPCHK(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
+	d->misplaced_seen = 0;
 	return 0;
 }
 

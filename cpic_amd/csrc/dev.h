@@ -182,6 +182,7 @@ struct cpic_dev {
 	int64_t iter;            // host mirror of the step counter (next step)
 	int64_t last_iter;       // last iteration pushed
 	int64_t nrebuilds;
+	unsigned long long misplaced_seen;   // ctr->misplaced already acted on
 
 	// captured steps: a graph of two consecutive steps (leaver parity and
 	// rho buffer alternate each step), valid for steps >= 1

@@ -465,7 +465,10 @@ def test_long_run_conservation_with_fast_beams(precision):
     """3,000 steps of fast beams (0.5 cell per step) on 8x8 tiles, replayed
     from captured graphs with a re-sort after step 1,500: the particle count
     of each species, the total charge sum(rho) = N and finiteness hold at
-    every check; leavers cross tiles every step."""
+    every check; leavers cross tiles every step.  About 7 % of the particles
+    leave their tile per step, more than the initial leaver buffer (N/32)
+    holds: the first sync grows it and re-sorts, so the second 500 steps
+    misplace far fewer particles than the first."""
     n, L = 262144, (128.0, 128.0)
     m = 2 * n / (L[0] * L[1])
     parts = two_stream(n, L, 5.0, 0.5, seed=41)
@@ -475,10 +478,12 @@ def test_long_run_conservation_with_fast_beams(precision):
     d.pre_step()
     it = 0
     tol = 1e-9 if precision == 64 else 1e-5
+    misplaced = []
     for block in range(6):
         d.run(it, 500)
         it += 500
         d.sync()
+        misplaced.append(d.stats()["misplaced"])
         assert d.count(0) == n and d.count(1) == n, it
         rho = d.field("rho")
         assert np.all(np.isfinite(rho)), it
@@ -487,6 +492,7 @@ def test_long_run_conservation_with_fast_beams(precision):
             d.rebuild()
     st = d.stats()
     assert st["leavers"] > 100 * n // 8
+    assert misplaced[0] > 0 and misplaced[1] - misplaced[0] < misplaced[0] // 4, misplaced
     W = d.energy()["field"][:it]
     assert np.all(np.isfinite(W))
     d.close()
