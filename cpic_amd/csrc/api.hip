@@ -571,6 +571,7 @@ commit(cpic_dev *d)
 		return ret;
 	HIPCHK(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
 	d->misplaced_seen = 0;
+	d->fallbacks_seen = 0;
 	return 0;
 }
 
@@ -580,8 +581,10 @@ commit(cpic_dev *d)
 // try to leave again every step, so the demand snowballs.  At a sync point
 // (between stages, no leaver in flight), when that has happened since the
 // last check, the buffer grows to 1.25x the last step's demand (at least
-// double, at most every particle) and the particles are re-sorted.  The
-// counters stay cumulative.
+// double, at most every particle) and the particles are re-sorted.  Leavers
+// that found their destination bucket full went back to their source bucket
+// (fallbacks) and retry likewise: a re-sort sizes the buckets for the
+// current counts.  The counters stay cumulative.
 static int
 heal_overflow(cpic_dev *d)
 {
@@ -589,8 +592,14 @@ heal_overflow(cpic_dev *d)
 	int ret;
 	HIPCHK(hipMemcpy(&c, d->ctr, sizeof(c), hipMemcpyDeviceToHost));
 	if(c.misplaced <= d->misplaced_seen)
-		return 0;
+	{
+		if(c.fallbacks <= d->fallbacks_seen)
+			return 0;
+		d->fallbacks_seen = c.fallbacks;
+		return cpic_parts_rebuild(d);
+	}
 	d->misplaced_seen = c.misplaced;
+	d->fallbacks_seen = c.fallbacks;
 	int64_t tot = 0;
 	for(int p = 0; p < d->P; p++)
 		for(int s = 0; s < d->hgeo[p].ns; s++)
@@ -707,6 +716,7 @@ cpic_dev_rebuild(cpic_dev_t *d)
 		return ret;
 	HIPCHK(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
 	d->misplaced_seen = 0;
+	d->fallbacks_seen = 0;
 	return 0;
 }
 

@@ -183,6 +183,7 @@ struct cpic_dev {
 	int64_t last_iter;       // last iteration pushed
 	int64_t nrebuilds;
 	unsigned long long misplaced_seen;   // ctr->misplaced already acted on
+	unsigned long long fallbacks_seen;   // ctr->fallbacks already acted on
 
 	// captured steps: a graph of two consecutive steps (leaver parity and
 	// rho buffer alternate each step), valid for steps >= 1
