@@ -14,7 +14,8 @@
 //            so the column pass reads contiguous columns), phi (double),
 //            E (double2 or float2 (Ex,Ey) pairs).
 //   particles buckets, one per (plasma, tile, species), tile-major inside a
-//            plasma; SoA arrays x, y, ux, uy (+ optional id, Ex, Ey).  A bucket
+//            plasma; state x, y, ux, uy in groups of 64 slots (+ optional id,
+//            Ex, Ey arrays).  A bucket
 //            owns slots [start, start+cap): segment 1 holds n1 particles at
 //            [0, n1), segment 2 holds n2 fallback particles at
 //            [cap-res, cap-res+n2), where res is the number of particles that

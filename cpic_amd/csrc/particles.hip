@@ -2,7 +2,8 @@
 //
 // k_push: one workgroup per tile.  Stages the tile's E region (tile + 1 cell
 // margin + the CIC +1 node) in LDS, streams every particle of the tile's
-// buckets once (coalesced SoA loads), and for each particle
+// buckets once (coalesced loads of the state, stored in groups of 64:
+// x[64] y[64] ux[64] uy[64]), and for each particle
 //   gathers E with CIC weights    (interpolate_f2p_E, src/interpolate.c:350-404)
 //   applies the Boris push        (boris_rotation/move, src/mover.c:23-83, 141-227)
 //   wraps periodically            (periodic_boundary_ppack, src/comm_plasma.c:726-747)
@@ -15,8 +16,8 @@
 //   buffers swap every step; each wavefront claims its stayers' slots with
 //   one LDS atomic, so no block-wide barrier is needed per chunk) while
 //   leavers (particles whose cell moved to another tile: comm_plasma's
-//   collect, src/comm_plasma.c:101-165) are appended to a leaver buffer
-//   through one wave-aggregated atomic per wavefront (ballot + popcount).
+//   collect, src/comm_plasma.c:101-165) wait in an LDS stage and are copied
+//   to a leaver buffer with one global atomic per bucket.
 // The LDS rho region is flushed once per tile with FP64 global atomics.
 // k_append then places each leaver at the end of its destination bucket
 // (inject_particles, src/comm_plasma.c:358-400) or, when that bucket is full,
@@ -428,7 +429,7 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 }
 
 // Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
-// thread tid takes base + u*CPIC_BLOCK + tid (coalesced SoA loads).
+// thread tid takes base + u*CPIC_BLOCK + tid (coalesced loads).
 template <typename T, bool IDS, int U>
 static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
 		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
