@@ -731,6 +731,22 @@ want_iter(cpic_dev *d, int64_t iter)
 	return cpic_set_iter(d, iter);
 }
 
+// The E array is brought up to date with phi only when something reads or
+// writes it, or before phi changes: field_E leaves it stale and the push
+// computes the E of its tiles from phi with the same expression.
+static int
+ensure_E(cpic_dev *d)
+{
+	int ret;
+	if(!d->E_stale)
+		return 0;
+	set_device(d);
+	if((ret = cpic_efield(d)))
+		return ret;
+	d->E_stale = 0;
+	return 0;
+}
+
 extern "C" int
 cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 {
@@ -742,9 +758,10 @@ cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 	set_device(d);
 	if((ret = want_iter(d, iter)))
 		return ret;
-	if((ret = cpic_solve(d, 1)))
+	if((ret = cpic_solve_step(d)))
 		return ret;
-	return cpic_efield(d);
+	d->E_stale = 1;
+	return 0;
 }
 
 extern "C" int
@@ -754,6 +771,8 @@ cpic_dev_solve_xy(cpic_dev_t *d)
 	if(!d)
 		return CPIC_EINVAL;
 	if((ret = commit(d)))
+		return ret;
+	if((ret = ensure_E(d)))
 		return ret;
 	set_device(d);
 	return cpic_solve(d, 0);
@@ -769,7 +788,11 @@ cpic_dev_plasma_E(cpic_dev_t *d, int64_t iter)
 	if((ret = commit(d)))
 		return ret;
 	if(d->flags & CPIC_F_PARTICLE_E)
+	{
+		if((ret = ensure_E(d)))
+			return ret;
 		return cpic_parts_gather(d);
+	}
 	return 0;
 }
 
@@ -969,6 +992,9 @@ cpic_batch_download_field(cpic_dev_t *d, int32_t p, int32_t which, double *out)
 	else if(which == CPIC_EX || which == CPIC_EY)
 	{
 		int c = which == CPIC_EX ? 0 : 1;
+		int ret = ensure_E(d);
+		if(ret)
+			return ret;
 		if(d->prec == 64)
 		{
 			double2 *h = (double2 *) malloc(sizeof(double2) * n);
@@ -1015,6 +1041,13 @@ cpic_batch_upload_field(cpic_dev_t *d, int32_t p, int32_t which, const double *i
 	n = (int64_t) d->hgeo[p].nx * d->hgeo[p].ny;
 	o = d->hgeo[p].node0;
 	set_device(d);
+	// phi and E are about to diverge: E keeps the value field_E gave it
+	if(which == CPIC_PHI || which == CPIC_EX || which == CPIC_EY)
+	{
+		int ret = ensure_E(d);
+		if(ret)
+			return ret;
+	}
 	HIPCHK(hipStreamSynchronize(d->stream));
 	if(which == CPIC_RHO)
 		HIPCHK(hipMemcpy(d->rho[d->rho_cur] + o, in, sizeof(double) * n, hipMemcpyHostToDevice));

@@ -151,6 +151,8 @@ struct cpic_dev {
 	double *rho[2];
 	int rho_cur;
 	int rho_next_ready;      // rho[1-cur] has been zeroed
+	int E_stale;             // phi is newer than the E array (field_E leaves
+	                         // E to the push; cpic_efield brings it up to date)
 	int rho_pending;         // plasma_r deposited into rho[1-cur]
 	double2 *ghat;
 	double *phi;
@@ -210,6 +212,7 @@ int cpic_fft_blocking_ok(const Geo *g);
 size_t cpic_push_lds(const cpic_dev *d);
 int cpic_set_iter(cpic_dev *d, int64_t iter);
 int cpic_solve(cpic_dev *d, int want_energy);
+int cpic_solve_step(cpic_dev *d);
 int cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi);
 int cpic_efield_ptr(cpic_dev *d, const double *phi, void *E);
 int cpic_efield(cpic_dev *d);

@@ -367,7 +367,8 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 __global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
 void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const double2 *__restrict__ ghat_all, double *__restrict__ phi_all,
-		const double2 *__restrict__ tw, int lgN)
+		const double2 *__restrict__ tw, int lgN, double *__restrict__ rho_zero_all,
+		DiagSlot *__restrict__ diag, int ring, const StepState *__restrict__ st)
 {
 	extern __shared__ double2 sh[];
 	int p = find_plasma(pre, P, blockIdx.x);
@@ -435,6 +436,22 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		double *row = phi + (int64_t) (y0 + r) * nx;
 		row[2 * j] = z.x;
 		row[2 * j + 1] = z.y;
+	}
+	// the same rows of the next rho buffer, which the coming push deposits
+	// into, and the next step's diagnostic slot start at zero
+	if(rho_zero_all)
+	{
+		double *__restrict__ rz = rho_zero_all + g.node0 + (int64_t) y0 * nx;
+		for(int i = threadIdx.x; i < R * nx; i += blockDim.x)
+			rz[i] = 0.0;
+	}
+	if(diag && blockIdx.x == pre[p] && threadIdx.x == 0)
+	{
+		DiagSlot *z = &diag[g.diag0 + (st->iter + 1) % ring];
+		z->field = 0.0;
+		z->kinetic = 0.0;
+		z->px = 0.0;
+		z->py = 0.0;
 	}
 }
 
@@ -625,7 +642,8 @@ cpic_fft_blocking_ok(const Geo *g)
 // state's own buffers); with diag, the field energy of each plasma goes to
 // its slot of the current step.
 static int
-solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag)
+solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag, double *rho_zero,
+		DiagSlot *diag_zero)
 {
 	int slot;
 	const int P = d->P;
@@ -646,7 +664,8 @@ solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag)
 
 	slot = cpic_kt_begin(d, "fft_rows_c2r");
 	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(FFT_THREADS), d->lds_rows, d->stream,
-			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, phi, d->tw, d->lgN);
+			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, phi, d->tw, d->lgN, rho_zero,
+			diag_zero, d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -656,13 +675,30 @@ int
 cpic_solve(cpic_dev *d, int want_energy)
 {
 	DiagSlot *diag = (want_energy && !(d->flags & CPIC_F_NO_DIAG)) ? d->diag : (DiagSlot *) NULL;
-	return solve(d, d->rho[d->rho_cur], d->phi, diag);
+	return solve(d, d->rho[d->rho_cur], d->phi, diag, (double *) NULL, (DiagSlot *) NULL);
+}
+
+// field_E's solve: also zeroes the rho buffer the next push deposits into
+// (unless already zero or holding a pending deposit) and the next step's
+// diagnostic slot, in the inverse row pass.  E is then computed from phi by
+// the push itself (or by cpic_efield when something reads E).
+int
+cpic_solve_step(cpic_dev *d)
+{
+	double *rz = NULL;
+	DiagSlot *diag = (d->flags & CPIC_F_NO_DIAG) ? (DiagSlot *) NULL : d->diag;
+	if(!d->rho_next_ready && !d->rho_pending)
+		rz = d->rho[1 - d->rho_cur];
+	int ret = solve(d, d->rho[d->rho_cur], d->phi, diag, rz, diag);
+	if(!ret && rz)
+		d->rho_next_ready = 1;
+	return ret;
 }
 
 int
 cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi)
 {
-	return solve(d, rho, phi, (DiagSlot *) NULL);
+	return solve(d, rho, phi, (DiagSlot *) NULL, (double *) NULL, (DiagSlot *) NULL);
 }
 
 int
@@ -681,6 +717,8 @@ cpic_efield_ptr(cpic_dev *d, const double *phi, void *E)
 	return 0;
 }
 
+// E from phi into the E array (for the callers that read E; the step's push
+// computes its tiles' E from phi itself).
 int
 cpic_efield(cpic_dev *d)
 {
@@ -689,10 +727,6 @@ cpic_efield(cpic_dev *d)
 	int slot;
 	const int P = d->P;
 
-	if(!d->rho_next_ready && !d->rho_pending)
-		rz = d->rho[1 - d->rho_cur];
-	if(!(d->flags & CPIC_F_NO_DIAG))
-		dz = d->diag;
 	slot = cpic_kt_begin(d, "efield");
 	if(d->prec == 64)
 		hipLaunchKernelGGL(k_efield<double2>, dim3(d->nodeblocks), dim3(CPIC_BLOCK), 0, d->stream,
@@ -704,7 +738,5 @@ cpic_efield(cpic_dev *d)
 				d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
-	if(rz)
-		d->rho_next_ready = 1;
 	return 0;
 }

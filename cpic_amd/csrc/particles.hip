@@ -1,7 +1,10 @@
 // Particle kernels of the gfx950 PIC step.
 //
 // k_push: one workgroup per tile.  Stages the tile's E region (tile + 1 cell
-// margin + the CIC +1 node) in LDS, streams every particle of the tile's
+// margin + the CIC +1 node) in LDS -- after a field solve it computes that
+// region from the potential (E = -grad phi, src/field.c:381-406, staged with a
+// one-node halo), so a step never writes or reads a global E array -- streams
+// every particle of the tile's
 // buckets once (coalesced loads of the state, stored in groups of 64:
 // x[64] y[64] ux[64] uy[64]), and for each particle
 //   gathers E with CIC weights    (interpolate_f2p_E, src/interpolate.c:350-404)
@@ -61,6 +64,7 @@ struct PushArgs {
 	int par;              // leaver counter parity
 	int first;            // iteration 0: dt' = -dt/2 and no position update
 	const void *E;        // whole batch, plasma p at node0
+	const double *phi;    // potential, whole batch (pushes that compute E)
 	double *rho;          // deposit target, whole batch
 	DiagSlot *diag;       // nullable: per-plasma rings of `ring` slots
 	int ring;
@@ -212,12 +216,13 @@ struct CellPos {
 };
 
 // Gather E at one particle: LDS region when the 2x2 stencil is inside it,
-// else straight from HBM.  Summation order of interpolate_f2p
-// (src/interpolate.c:151-154): w00, w01, w10, w11 with fma accumulation.
-template <typename T>
+// else straight from HBM (the E array, or with EPHI the potential).
+// Summation order of interpolate_f2p (src/interpolate.c:151-154): w00, w01,
+// w10, w11 with fma accumulation.
+template <typename T, bool EPHI>
 static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>::t *sE,
-		const typename V2<T>::t *__restrict__ E, const CellPos &c, T rx, T ry,
-		T &ex, T &ey)
+		const typename V2<T>::t *__restrict__ E, const double *__restrict__ phi,
+		const CellPos &c, T rx, T ry, T &ex, T &ey)
 {
 	typedef typename V2<T>::t T2;
 	T dxw = (T) 1 - rx, dyw = (T) 1 - ry;
@@ -240,6 +245,41 @@ static __device__ __forceinline__ void gather(const Geo &g, const typename V2<T>
 		e10.y = a10.y;
 		e11.x = a11.x;
 		e11.y = a11.y;
+	}
+	else if(EPHI)
+	{
+		// rare (particles left outside their tile's region by a full
+		// leaver buffer); one division at a time keeps the register demand
+		// of the particle loop down.  E at a node as k_efield computes it
+		// (src/field.c:381-406: central differences over 2 dx and 2 dy),
+		// same products and summation order as below.
+		int i1x = c.ix + 1 == g.nx ? 0 : c.ix + 1;
+		int i1y = c.iy + 1 == g.ny ? 0 : c.iy + 1;
+		T acc[2];
+#pragma unroll 1
+		for(int j = 0; j < 8; j++)
+		{
+			int k = j & 3, comp = j >> 2;
+			int ix = k & 2 ? i1x : c.ix, iy = k & 1 ? i1y : c.iy;
+			int64_t n0, n1;
+			if(comp)
+			{
+				n0 = (int64_t) (iy == 0 ? g.ny - 1 : iy - 1) * g.nx + ix;
+				n1 = (int64_t) (iy == g.ny - 1 ? 0 : iy + 1) * g.nx + ix;
+			}
+			else
+			{
+				n0 = (int64_t) iy * g.nx + (ix == 0 ? g.nx - 1 : ix - 1);
+				n1 = (int64_t) iy * g.nx + (ix == g.nx - 1 ? 0 : ix + 1);
+			}
+			T v = (T) ((phi[n0] - phi[n1]) / (comp ? 2 * g.dy : 2 * g.dx));
+			T w = k == 0 ? w00 : k == 1 ? w01 : k == 2 ? w10 : w11;
+			T a = k == 0 ? w * v : fma(w, v, acc[comp]);
+			acc[comp] = a;
+		}
+		ex = acc[0];
+		ey = acc[1];
+		return;
 	}
 	else
 	{
@@ -344,6 +384,40 @@ static __device__ __forceinline__ void tile_setup(const Geo &g, int ox, int oy,
 	}
 }
 
+// Computes the tile's E region in LDS from the potential: the region plus a
+// one-node halo of phi is staged in the rho area (row stride W + 2), E
+// follows with the expression of k_efield (src/field.c:381-406), then the rho
+// area is zeroed.
+// The grid sides are powers of two, so "& (n - 1)" wraps any offset.
+template <typename T>
+static __device__ __forceinline__ void tile_setup_phi(const Geo &g, int ox, int oy,
+		const double *__restrict__ phi, typename V2<T>::t *sE, double *sR)
+{
+	typedef typename V2<T>::t T2;
+	const int PW = g.W + 2, PH = g.H + 2;
+	for(int i = threadIdx.x; i < PW * PH; i += CPIC_BLOCK)
+	{
+		int ly = i / PW, lx = i - ly * PW;
+		int gx = (ox - 1 + lx) & (g.nx - 1), gy = (oy - 1 + ly) & (g.ny - 1);
+		sR[i] = phi[(int64_t) gy * g.nx + gx];
+	}
+	__syncthreads();
+	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	{
+		int ly = i / g.W, lx = i - ly * g.W;
+		int c = (ly + 1) * PW + lx + 1;
+		double ey = (sR[c - PW] - sR[c + PW]) / (2 * g.dy);
+		double ex = (sR[c - 1] - sR[c + 1]) / (2 * g.dx);
+		T2 v;
+		v.x = ex;
+		v.y = ey;
+		sE[i] = v;
+	}
+	__syncthreads();
+	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+		sR[i] = 0.0;
+}
+
 // Adds the LDS rho region into the global rho with FP64 atomics.
 template <typename T>
 static __device__ __forceinline__ void tile_flush(const Geo &g, int ox, int oy, const double *sR,
@@ -363,9 +437,10 @@ static __device__ __forceinline__ void tile_flush(const Geo &g, int ox, int oy, 
 
 // One particle: gather, Boris push, move + wrap, deposit at the new
 // position; returns whether its cell left the tile (and the destination).
-template <typename T>
+template <typename T, bool EPHI>
 static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<T>::t *sE,
-		const typename V2<T>::t *__restrict__ E, double *sR, double *__restrict__ rho,
+		const typename V2<T>::t *__restrict__ E, const double *__restrict__ phi, double *sR,
+		double *__restrict__ rho,
 		int tile, int ox, int oy, int s, const SpeciesK<T> &k, bool set_r, double dtp,
 		float dtdx, float dtdy, PState<T> &q, int &ndst, double &ke, double &mpx,
 		double &mpy, bool &bad, bool &nan_)
@@ -375,7 +450,7 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 	CellPos c;
 	T rx, ry, ex, ey;
 	locate(g, ox, oy, q.x, q.y, c, rx, ry);
-	gather<T>(g, sE, E, c, rx, ry, ex, ey);
+	gather<T, EPHI>(g, sE, E, phi, c, rx, ry, ex, ey);
 	q.Ex = (double) ex;
 	q.Ey = (double) ey;
 	T unx = q.ux, uny = q.uy;
@@ -499,7 +574,8 @@ static __device__ __forceinline__ void back_to_tile(const Geo &g, int ndst, int 
 	q.y = (T) ((float) q.y + (float) dty);
 }
 
-template <typename T, bool IDS, bool PE, int U>
+// EPHI: the tile's E comes from the potential (the E array is stale).
+template <typename T, bool IDS, bool PE, int U, bool EPHI>
 __global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
 {
@@ -526,12 +602,16 @@ void k_push(PushArgs a)
 	int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	unsigned long long lt = (1ull << lane) - 1ull;
 	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
+	const double *__restrict__ phi = EPHI ? a.phi + g.node0 : (const double *) NULL;
 	double *__restrict__ rho = a.rho + g.node0;
 	const double dtp = a.first ? -g.dt / 2 : g.dt;
 	const bool set_r = !a.first;
 	const float dtdx = (float) (dtp / g.dx), dtdy = (float) (dtp / g.dy);
 
-	tile_setup<T>(g, ox, oy, E, sE, sR);
+	if(EPHI)
+		tile_setup_phi<T>(g, ox, oy, phi, sE, sR);
+	else
+		tile_setup<T>(g, ox, oy, E, sE, sR);
 	if(tid == 0)
 	{
 		s_kcount = 0;
@@ -566,7 +646,7 @@ void k_push(PushArgs a)
 				leave[u] = false;
 				ndst[u] = 0;
 				if(act[u])
-					leave[u] = push_one<T>(g, sE, E, sR, rho, tile, ox, oy, s, sk, set_r,
+					leave[u] = push_one<T, EPHI>(g, sE, E, phi, sR, rho, tile, ox, oy, s, sk, set_r,
 							dtp, dtdx, dtdy, q[u], ndst[u], ke, mpx, mpy, bad, nan_);
 			}
 			// leavers: staged in LDS (one LDS atomic per wavefront); once
@@ -802,7 +882,7 @@ void k_gather(PushArgs a)
 			CellPos c;
 			T rx, ry, ex, ey;
 			locate(g, ox, oy, p_x<T>(a.p, slot), p_y<T>(a.p, slot), c, rx, ry);
-			gather<T>(g, sE, E, c, rx, ry, ex, ey);
+			gather<T, false>(g, sE, E, (const double *) NULL, c, rx, ry, ex, ey);
 			a.p.Ex[slot] = (double) ex;
 			a.p.Ey[slot] = (double) ey;
 		}
@@ -1172,19 +1252,22 @@ void k_cell_counts(Geo g, Buckets bk, Parts p, int s, unsigned long long *__rest
 		else hipLaunchKernelGGL((kern<T, false, false>), __VA_ARGS__); \
 	} while(0)
 
-#define DISPATCH_UU(kern, T, U, ...) do { \
+#define DISPATCH_UU(kern, T, U, EP, ...) do { \
 		bool ids_ = (d->flags & CPIC_F_IDS) != 0; \
 		bool pe_ = (d->flags & CPIC_F_PARTICLE_E) != 0; \
-		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, U>), __VA_ARGS__); \
-		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, U>), __VA_ARGS__); \
-		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, U>), __VA_ARGS__); \
-		else hipLaunchKernelGGL((kern<T, false, false, U>), __VA_ARGS__); \
+		if(ids_ && pe_) hipLaunchKernelGGL((kern<T, true, true, U, EP>), __VA_ARGS__); \
+		else if(ids_) hipLaunchKernelGGL((kern<T, true, false, U, EP>), __VA_ARGS__); \
+		else if(pe_) hipLaunchKernelGGL((kern<T, false, true, U, EP>), __VA_ARGS__); \
+		else hipLaunchKernelGGL((kern<T, false, false, U, EP>), __VA_ARGS__); \
 	} while(0)
 
-// particles per thread and chunk: 2 (default) or 4, chosen per handle
+// particles per thread and chunk: 2 (default) or 4, chosen per handle; E
+// from the potential when the E array is stale
 #define DISPATCH_U(kern, T, ...) do { \
-		if(d->ppt == 4) DISPATCH_UU(kern, T, 4, __VA_ARGS__); \
-		else DISPATCH_UU(kern, T, 2, __VA_ARGS__); \
+		if(d->ppt == 4 && d->E_stale) DISPATCH_UU(kern, T, 4, true, __VA_ARGS__); \
+		else if(d->ppt == 4) DISPATCH_UU(kern, T, 4, false, __VA_ARGS__); \
+		else if(d->E_stale) DISPATCH_UU(kern, T, 2, true, __VA_ARGS__); \
+		else DISPATCH_UU(kern, T, 2, false, __VA_ARGS__); \
 	} while(0)
 
 #define DISPATCH_P(kern, ...) do { \
@@ -1214,12 +1297,21 @@ max_wh(const cpic_dev *d)
 	return m;
 }
 
+// E region plus the rho region, which also holds the potential with its
+// one-node halo while a push computes E from it
 size_t
 cpic_push_lds(const cpic_dev *d)
 {
-	size_t n = max_wh(d);
-	return d->prec == 64 ? n * (sizeof(double2) + sizeof(double))
-		: n * (sizeof(float2) + sizeof(double));
+	size_t n = 0, h = 0;
+	for(int p = 0; p < d->P; p++)
+	{
+		size_t w = (size_t) d->hgeo[p].W, hh = (size_t) d->hgeo[p].H;
+		if(w * hh > n)
+			n = w * hh;
+		if((w + 2) * (hh + 2) > h)
+			h = (w + 2) * (hh + 2);
+	}
+	return n * (d->prec == 64 ? sizeof(double2) : sizeof(float2)) + h * sizeof(double);
 }
 
 void
@@ -1271,6 +1363,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	a.par = d->step_parity;
 	a.first = first;
 	a.E = d->E;
+	a.phi = d->phi;
 	a.rho = rho;
 	a.diag = (d->flags & CPIC_F_NO_DIAG) ? (DiagSlot *) NULL : d->diag;
 	a.ring = d->ring;
@@ -1672,14 +1765,22 @@ cpic_parts_init(cpic_dev *d)
 {
 	(void) d;
 	const void *k[] = {
-		(const void *) k_push<double, false, false, 2>, (const void *) k_push<double, true, false, 2>,
-		(const void *) k_push<double, false, true, 2>, (const void *) k_push<double, true, true, 2>,
-		(const void *) k_push<float, false, false, 2>, (const void *) k_push<float, true, false, 2>,
-		(const void *) k_push<float, false, true, 2>, (const void *) k_push<float, true, true, 2>,
-		(const void *) k_push<double, false, false, 4>, (const void *) k_push<double, true, false, 4>,
-		(const void *) k_push<double, false, true, 4>, (const void *) k_push<double, true, true, 4>,
-		(const void *) k_push<float, false, false, 4>, (const void *) k_push<float, true, false, 4>,
-		(const void *) k_push<float, false, true, 4>, (const void *) k_push<float, true, true, 4>,
+		(const void *) k_push<double, false, false, 2, false>, (const void *) k_push<double, true, false, 2, false>,
+		(const void *) k_push<double, false, true, 2, false>, (const void *) k_push<double, true, true, 2, false>,
+		(const void *) k_push<float, false, false, 2, false>, (const void *) k_push<float, true, false, 2, false>,
+		(const void *) k_push<float, false, true, 2, false>, (const void *) k_push<float, true, true, 2, false>,
+		(const void *) k_push<double, false, false, 4, false>, (const void *) k_push<double, true, false, 4, false>,
+		(const void *) k_push<double, false, true, 4, false>, (const void *) k_push<double, true, true, 4, false>,
+		(const void *) k_push<float, false, false, 4, false>, (const void *) k_push<float, true, false, 4, false>,
+		(const void *) k_push<float, false, true, 4, false>, (const void *) k_push<float, true, true, 4, false>,
+		(const void *) k_push<double, false, false, 2, true>, (const void *) k_push<double, true, false, 2, true>,
+		(const void *) k_push<double, false, true, 2, true>, (const void *) k_push<double, true, true, 2, true>,
+		(const void *) k_push<float, false, false, 2, true>, (const void *) k_push<float, true, false, 2, true>,
+		(const void *) k_push<float, false, true, 2, true>, (const void *) k_push<float, true, true, 2, true>,
+		(const void *) k_push<double, false, false, 4, true>, (const void *) k_push<double, true, false, 4, true>,
+		(const void *) k_push<double, false, true, 4, true>, (const void *) k_push<double, true, true, 4, true>,
+		(const void *) k_push<float, false, false, 4, true>, (const void *) k_push<float, true, false, 4, true>,
+		(const void *) k_push<float, false, true, 4, true>, (const void *) k_push<float, true, true, 4, true>,
 		(const void *) k_deposit<double>, (const void *) k_deposit<float>,
 		(const void *) k_gather<double>, (const void *) k_gather<float>,
 	};

@@ -94,10 +94,12 @@ def check_plasma(b, p, c, o, W, tol):
     assert rel_l2(e["field"], W) < max(tol, 1e-12), (p, rel_l2(e["field"], W))
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_fp64_each_plasma_matches_oracle(graphs):
+@pytest.mark.parametrize("graphs,particle_e", [(True, True), (False, True), (True, False)])
+def test_batch_fp64_each_plasma_matches_oracle(graphs, particle_e):
+    """particle_e False: the push computes E from phi in every plasma's
+    tiles (regions of different sizes in one launch)."""
     cfgs = plasmas()
-    flags = FLAGS | (0 if graphs else ca.F_NO_GRAPH)
+    flags = ca.F_IDS | (ca.F_PARTICLE_E if particle_e else 0) | (0 if graphs else ca.F_NO_GRAPH)
     b = make_batch(cfgs, flags=flags)
     assert b.P == 4
     b.pre_step()

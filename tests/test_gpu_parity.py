@@ -162,6 +162,49 @@ def test_20_steps_chunks_and_tight_capacity():
     assert_same_particles(o, d, 2, exact=False, tol=1e-12)
 
 
+@pytest.mark.parametrize("tile,ppt", [((16, 16), 2), ((8, 8), 4), ((64, 64), 2)])
+def test_20_steps_push_computes_E_from_phi(tile, ppt):
+    """Without per-particle E output nothing reads the E array during a step,
+    so the push computes its tiles' E from phi: the same 1e-12 bar over 20
+    steps, and E read afterwards (computed on demand) matches the oracle's."""
+    o, d = c1(seed=4, tile=tile, flags=ca.F_IDS, ppt=ppt)
+    W = run_both(o, d, 20)
+    d.sync()
+    for f in ("rho", "phi", "Ex", "Ey"):
+        assert rel_l2(d.field(f), o.field(f)) < 1e-12, f
+    assert_same_particles(o, d, 2, exact=False, tol=1e-12)
+    assert rel_l2(d.energy()["field"], W) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_push_E_from_phi_bitwise_equals_E_array(precision):
+    """One step from a shared rho gives the same particles bit for bit
+    whether the push reads the E array (per-particle E output requested),
+    computes E from phi, or phi is overwritten between field_E and the push
+    (E keeps the value field_E gave it, as in the reference)."""
+    got = []
+    rho = None
+    for flags, clobber in ((ca.F_IDS | ca.F_PARTICLE_E, False), (ca.F_IDS, False),
+                           (ca.F_IDS, True)):
+        o, d = c1(seed=8, tile=(8, 8), flags=flags, precision=precision)
+        o.pre_step()
+        d.pre_step()
+        if rho is None:
+            rho = o.field("rho")
+        d.set_field("rho", rho)
+        d.stage("field_E", 1)
+        if clobber:
+            d.set_field("phi", np.zeros_like(rho))
+        d.stage("plasma_E", 1)
+        d.stage("plasma_r", 1)
+        got.append([d.particles(s) for s in range(2)])
+        d.close()
+    for run in got[1:]:
+        for s in range(2):
+            for k in ("id", "x", "y", "ux", "uy"):
+                assert np.array_equal(run[s][k], got[0][s][k]), (s, k)
+
+
 def test_rebuild_midway():
     o, d = c1(seed=6)
     o.pre_step()
