@@ -144,6 +144,34 @@ static __device__ __forceinline__ int64_t pslot(int64_t s)
 	return ((s >> 6) << 8) + (s & 63);
 }
 
+// Bucket starts are multiples of PGROUP (capacities are rounded up to it),
+// so slot bstart + j sits at element 4 * bstart + pslot(j): the push keeps a
+// per-bucket base and does 32-bit index work per particle.
+template <typename T>
+static __device__ __forceinline__ const GLB_AS T *bucket_base(const Parts &p, int64_t bstart)
+{
+	return gptr<T>(p.r) + 4 * bstart;
+}
+
+static __device__ __forceinline__ int64_t pslot_rel(int j)
+{
+	return ((int64_t) (j >> 6) << 8) + (j & 63);
+}
+
+// A block-uniform value read from memory the kernel also writes: the
+// compiler must use a vector load, so the value would occupy VGPRs.
+static __device__ __forceinline__ int uniform(int v)
+{
+	return __builtin_amdgcn_readfirstlane(v);
+}
+
+static __device__ __forceinline__ int64_t uniform(int64_t v)
+{
+	unsigned lo = (unsigned) __builtin_amdgcn_readfirstlane((int) (uint64_t) v);
+	unsigned hi = (unsigned) __builtin_amdgcn_readfirstlane((int) ((uint64_t) v >> 32));
+	return (int64_t) (((uint64_t) hi << 32) | lo);
+}
+
 template <typename T>
 static __device__ __forceinline__ T p_x(const Parts &p, int64_t s)
 {
@@ -509,14 +537,24 @@ template <typename T, bool IDS, int U>
 static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
 		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
 {
+	const GLB_AS T *pb = bucket_base<T>(p, bstart);
 #pragma unroll
 	for(int u = 0; u < U; u++)
 	{
 		int j = base + u * CPIC_BLOCK + tid;
 		act[u] = j < ntot;
-		int64_t slot = bstart + j + (j >= n1 ? seg2 : 0);
+		int jj = j + (j >= n1 ? seg2 : 0);
 		if(act[u])
-			load_p<T>(p, slot, q[u], IDS);
+		{
+			// global-address-space loads (see load_p)
+			const GLB_AS T *r = pb + pslot_rel(jj);
+			q[u].x = r[0];
+			q[u].y = r[PGROUP];
+			q[u].ux = r[2 * PGROUP];
+			q[u].uy = r[3 * PGROUP];
+			if(IDS)
+				q[u].id = gptr<int64_t>(p.id)[bstart + jj];
+		}
 	}
 }
 
@@ -627,10 +665,10 @@ void k_push(PushArgs a)
 	for(int s = 0; s < g.ns; s++)
 	{
 		const int b = g.bucket0 + tile * g.ns + s;
-		const int64_t bstart = a.bk.start[b];
-		const int cap = a.bk.cap[b], res = a.bk.res[b];
-		const int n1 = min(a.bk.n1[b], cap - res);
-		const int ntot = n1 + a.bk.n2[b];
+		const int64_t bstart = uniform(a.bk.start[b]);
+		const int cap = uniform(a.bk.cap[b]), res = uniform(a.bk.res[b]);
+		const int n1 = min(uniform(a.bk.n1[b]), cap - res);
+		const int ntot = n1 + uniform(a.bk.n2[b]);
 		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
 		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 
@@ -723,7 +761,21 @@ void k_push(PushArgs a)
 						wb = atomicAdd(&s_w, __popcll(smk));
 					wb = __builtin_amdgcn_readfirstlane(wb);
 					if(act[u] && !leave[u])
-						store_p<T>(a.q, bstart + wb + __popcll(smk & lt), q[u], IDS, PE);
+					{
+						int jj = wb + __popcll(smk & lt);
+						GLB_AS T *r = (GLB_AS T *) bucket_base<T>(a.q, bstart) + pslot_rel(jj);
+						r[0] = q[u].x;
+						r[PGROUP] = q[u].y;
+						r[2 * PGROUP] = q[u].ux;
+						r[3 * PGROUP] = q[u].uy;
+						if(IDS)
+							a.q.id[bstart + jj] = q[u].id;
+						if(PE)
+						{
+							a.q.Ex[bstart + jj] = q[u].Ex;
+							a.q.Ey[bstart + jj] = q[u].Ey;
+						}
+					}
 				}
 			}
 		}
@@ -1601,8 +1653,9 @@ cpic_parts_rebuild(cpic_dev *d)
 			slack = 0.0;
 			pad = 0;
 		}
+		// capacities in whole groups: bucket starts are multiples of PGROUP
 		hipLaunchKernelGGL(k_scan_caps, dim3(1), dim3(1024), 0, d->stream, cnt, nb, slack, pad,
-				32, nstart, ncap, total);
+				PGROUP, nstart, ncap, total);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipMemcpyAsync(&tot, total, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
 		HIPCHK(hipStreamSynchronize(d->stream));
