@@ -567,9 +567,10 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 // to the leaver buffer with one atomic per wavefront.
 //
 // Waves per SIMD the register allocation of k_push must allow.  Measured at
-// 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (78 VGPRs with
-// U = 2), the FP64 kernel loses from any bound (its state is twice the size);
-// bounding the FP32 U = 4 kernel (95 VGPRs) to 6 waves spills and loses 12 %.
+// 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (U = 2, now 73
+// VGPRs; 7 waves gain nothing), the FP64 kernel loses from any bound (its
+// state is twice the size); bounding the FP32 U = 4 kernel (then 95 VGPRs,
+// now 91) to 6 waves spilled and lost 12 %.
 #ifndef PUSH_MIN_WAVES_F32
 #define PUSH_MIN_WAVES_F32 6
 #endif
