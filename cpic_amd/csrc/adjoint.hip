@@ -99,23 +99,55 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 #define ADJ_BOX_NODES 3072          // 24 KB of double
 #endif
 
+// LDS accumulation box of a block: nodes [x0, x0 + w) x [y0, y0 + h) in
+// coordinates relative to a reference cell (rx, ry), wrapped into
+// [-n/2, n/2).  A block's particles then form one compact box also where
+// they straddle the periodic boundary: a chunk of the cell-sorted order
+// that ends one grid row and begins the next, or particles that crossed
+// x = 0.  (With absolute coordinates such a box spans the whole row, and
+// the block fell back to global atomics on a few rows of nodes, which
+// serialise: 6-9 % of the blocks at 256^2.)
 struct Box {
 	int x0, y0, w, h;
+	int rx, ry;
 	bool lds;
 };
 
-// Bounding box (plus the CIC +1 node) of the block's cells [lo, hi].
-static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy, int *s4,
-		int cap)
+// grid sides are powers of two
+static __device__ __forceinline__ int rel_wrap(int i, int ref, int n)
+{
+	int r = (i - ref) & (n - 1);
+	return r >= n / 2 ? r - n : r;
+}
+
+// Bounding box (plus the CIC +1 node) of the block's cells; ix[u] < 0 marks
+// no particle.  Thread 0's first particle (the block's first, which always
+// exists) is the reference cell.  s6: six ints of LDS.
+template <int U>
+static __device__ Box block_box(const Geo &g, const int *ix, const int *iy, int *s6, int cap)
 {
 	if(threadIdx.x == 0)
 	{
-		s4[0] = 0x7fffffff;
-		s4[1] = -1;
-		s4[2] = 0x7fffffff;
-		s4[3] = -1;
+		s6[0] = 0x7fffffff;
+		s6[1] = -0x7fffffff;
+		s6[2] = 0x7fffffff;
+		s6[3] = -0x7fffffff;
+		s6[4] = ix[0];
+		s6[5] = iy[0];
 	}
 	__syncthreads();
+	const int rx = s6[4], ry = s6[5];
+	int lox = 0x7fffffff, hix = -0x7fffffff, loy = 0x7fffffff, hiy = -0x7fffffff;
+#pragma unroll
+	for(int u = 0; u < U; u++)
+		if(ix[u] >= 0)
+		{
+			int ax = rel_wrap(ix[u], rx, g.nx), ay = rel_wrap(iy[u], ry, g.ny);
+			lox = min(lox, ax);
+			hix = max(hix, ax);
+			loy = min(loy, ay);
+			hiy = max(hiy, ay);
+		}
 	// wavefront reduction first: four LDS atomics per wavefront
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -124,21 +156,22 @@ static __device__ Box block_box(const Geo &g, int lox, int hix, int loy, int hiy
 		loy = min(loy, __shfl_xor(loy, off));
 		hiy = max(hiy, __shfl_xor(hiy, off));
 	}
-	if((threadIdx.x & 63) == 0 && hix >= 0)
+	if((threadIdx.x & 63) == 0 && hix >= lox)
 	{
-		atomicMin(&s4[0], lox);
-		atomicMax(&s4[1], hix);
-		atomicMin(&s4[2], loy);
-		atomicMax(&s4[3], hiy);
+		atomicMin(&s6[0], lox);
+		atomicMax(&s6[1], hix);
+		atomicMin(&s6[2], loy);
+		atomicMax(&s6[3], hiy);
 	}
 	__syncthreads();
 	Box b;
-	b.x0 = s4[0];
-	b.y0 = s4[2];
-	b.w = s4[1] - s4[0] + 2;
-	b.h = s4[3] - s4[2] + 2;
-	b.lds = s4[1] >= 0 && s4[1] - s4[0] <= g.nx / 2 && s4[3] - s4[2] <= g.ny / 2 &&
-		b.w * b.h <= cap;
+	b.rx = rx;
+	b.ry = ry;
+	b.x0 = s6[0];
+	b.y0 = s6[2];
+	b.w = s6[1] - s6[0] + 2;
+	b.h = s6[3] - s6[2] + 2;
+	b.lds = s6[1] - s6[0] <= g.nx / 2 && s6[3] - s6[2] <= g.ny / 2 && b.w * b.h <= cap;
 	__syncthreads();
 	return b;
 }
@@ -149,10 +182,11 @@ static __device__ __forceinline__ void cell_of(const Geo &g, double x, double y,
 	cell64(g, x, y, ix, iy, rx, ry);
 }
 
-// node of the box (dx, dy in {0,1} from cell (ix, iy)), or the global node
-static __device__ __forceinline__ int box_node(const Box &b, int ix, int iy, int dx, int dy)
+// box node (dx, dy in {0,1} from cell (ix, iy))
+static __device__ __forceinline__ int box_node(const Geo &g, const Box &b, int ix, int iy,
+		int dx, int dy)
 {
-	return (iy - b.y0 + dy) * b.w + (ix - b.x0 + dx);
+	return (rel_wrap(iy, b.ry, g.ny) - b.y0 + dy) * b.w + (rel_wrap(ix, b.rx, g.nx) - b.x0 + dx);
 }
 
 static __device__ __forceinline__ int64_t wrap_node(const Geo &g, int x, int y)
@@ -173,10 +207,10 @@ static __device__ __forceinline__ void dep1(const Geo &g, const Box &b, double *
 	if(b.lds)
 	{
 		LDS_AS double *l = (LDS_AS double *) sacc;
-		lds_add(&l[box_node(b, ix, iy, 0, 0)], v00);
-		lds_add(&l[box_node(b, ix, iy, 0, 1)], v01);
-		lds_add(&l[box_node(b, ix, iy, 1, 0)], v10);
-		lds_add(&l[box_node(b, ix, iy, 1, 1)], v11);
+		lds_add(&l[box_node(g, b, ix, iy, 0, 0)], v00);
+		lds_add(&l[box_node(g, b, ix, iy, 0, 1)], v01);
+		lds_add(&l[box_node(g, b, ix, iy, 1, 0)], v10);
+		lds_add(&l[box_node(g, b, ix, iy, 1, 1)], v11);
 	}
 	else
 	{
@@ -206,7 +240,8 @@ static __device__ void box_flush(const Geo &g, const Box &b, const double *sacc,
 	for(int i = threadIdx.x; i < b.w * b.h; i += blockDim.x)
 	{
 		int ly = i / b.w, lx = i - ly * b.w;
-		int64_t gnode = wrap_node(g, b.x0 + lx, b.y0 + ly);
+		int gx = (b.rx + b.x0 + lx) & (g.nx - 1), gy = (b.ry + b.y0 + ly) & (g.ny - 1);
+		int64_t gnode = (int64_t) gy * g.nx + gx;
 		for(int k = 0; k < ncomp; k++)
 		{
 			double v = ((const LDS_AS double *) sacc)[i * ncomp + k];
@@ -222,11 +257,10 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 		const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ rho)
 {
 	__shared__ double sacc[ADJ_BOX_NODES];
-	__shared__ int s4[4];
+	__shared__ int s6[6];
 	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
 	int ix[ADJ_PPT], iy[ADJ_PPT];
 	double rx[ADJ_PPT], ry[ADJ_PPT];
-	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
 	{
@@ -235,13 +269,9 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 		if(i < n)
 		{
 			cell_of(g, x[i], y[i], ix[u], iy[u], rx[u], ry[u]);
-			lox = min(lox, ix[u]);
-			hix = max(hix, ix[u]);
-			loy = min(loy, iy[u]);
-			hiy = max(hiy, iy[u]);
 		}
 	}
-	Box b = block_box(g, lox, hix, loy, hiy, s4, ADJ_BOX_NODES);
+	Box b = block_box<ADJ_PPT>(g, ix, iy, s6, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
@@ -261,12 +291,11 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double *__restrict__ rho_next, int *derr)
 {
 	__shared__ double sacc[ADJ_BOX_NODES];
-	__shared__ int s4[4];
+	__shared__ int s6[6];
 	bool bad = false, nan_ = false;
 	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
 	int ix[ADJ_PPT], iy[ADJ_PPT];
 	double rx[ADJ_PPT], ry[ADJ_PPT];
-	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
 	{
@@ -312,16 +341,12 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		ux[i] = uu;
 		uy[i] = vv;
 		cell_of(g, px, py, ix[u], iy[u], rx[u], ry[u]);
-		lox = min(lox, ix[u]);
-		hix = max(hix, ix[u]);
-		loy = min(loy, iy[u]);
-		hiy = max(hiy, iy[u]);
 	}
 	if(bad)
 		atomicOr(derr, CPIC_DERR_UMAX);
 	if(nan_)
 		atomicOr(derr, CPIC_DERR_NAN);
-	Box b = block_box(g, lox, hix, loy, hiy, s4, ADJ_BOX_NODES);
+	Box b = block_box<ADJ_PPT>(g, ix, iy, s6, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
 	for(int u = 0; u < ADJ_PPT; u++)
@@ -391,11 +416,10 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double2 *__restrict__ lE)
 {
 	__shared__ double sacc[2 * REV_BOX_NODES];
-	__shared__ int s4[4];
+	__shared__ int s6[6];
 	int64_t p0 = (int64_t) blockIdx.x * REV_CHUNK;
 	int ix[REV_PPT], iy[REV_PPT];
 	double rx[REV_PPT], ry[REV_PPT], lex[REV_PPT], ley[REV_PPT];
-	int lox = 0x7fffffff, hix = -1, loy = 0x7fffffff, hiy = -1;
 #pragma unroll
 	for(int u = 0; u < REV_PPT; u++)
 	{
@@ -433,12 +457,8 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		double gyy = g.idy * (dxw * (e01.y - e00.y) + rx[u] * (e11.y - e10.y));
 		lx[i] = ax + (lex[u] * gxx + ley[u] * gxy);
 		ly[i] = ay + (lex[u] * gyx + ley[u] * gyy);
-		lox = min(lox, ix[u]);
-		hix = max(hix, ix[u]);
-		loy = min(loy, iy[u]);
-		hiy = max(hiy, iy[u]);
 	}
-	Box b = block_box(g, lox, hix, loy, hiy, s4, REV_BOX_NODES);
+	Box b = block_box<REV_PPT>(g, ix, iy, s6, REV_BOX_NODES);
 	box_zero(b, sacc, 2);
 #pragma unroll
 	for(int u = 0; u < REV_PPT; u++)
@@ -454,7 +474,7 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 			if(b.lds)
 			{
 				LDS_AS double *l = (LDS_AS double *) sacc;
-				int nd = box_node(b, ix[u], iy[u], ddx[c], ddy[c]);
+				int nd = box_node(g, b, ix[u], iy[u], ddx[c], ddy[c]);
 				lds_add(&l[2 * nd], w[c] * lex[u]);
 				lds_add(&l[2 * nd + 1], w[c] * ley[u]);
 			}
