@@ -23,6 +23,7 @@
 // the positions of its iterations.  No counterpart in the reference, which
 // has no gradients.
 #include "dev.h"
+#include "host_sort.h"
 #include "picmath.h"
 
 #include <cmath>
@@ -774,69 +775,12 @@ copy_state(cpic_adj *a, double *dst, const double *x, const double *y, const dou
 static int
 sort_by_cell(cpic_adj *a, const double *x, const double *y)
 {
-	const int64_t n = a->n, nc = (int64_t) a->g.nx * a->g.ny;
-	int64_t *start = (int64_t *) calloc((size_t) nc + 1, sizeof(int64_t));
-	int64_t *cell = (int64_t *) malloc(sizeof(int64_t) * (size_t) n);
-	if(!start || !cell)
-	{
-		free(start);
-		free(cell);
-		return CPIC_ENOMEM;
-	}
-	for(int64_t i = 0; i < n; i++)
-	{
-		int64_t ix = (int64_t) (x[i] * a->g.idx), iy = (int64_t) (y[i] * a->g.idy);
-		ix = ix < 0 ? 0 : (ix >= a->g.nx ? a->g.nx - 1 : ix);
-		iy = iy < 0 ? 0 : (iy >= a->g.ny ? a->g.ny - 1 : iy);
-		cell[i] = iy * a->g.nx + ix;
-		start[cell[i] + 1]++;
-	}
-	for(int64_t c = 0; c < nc; c++)
-		start[c + 1] += start[c];
-	for(int64_t i = 0; i < n; i++)
-		a->perm[start[cell[i]]++] = i;
-	free(start);
-	// Within each block's chunk (same cells, so the same LDS box), take the
-	// cells round-robin: the 64 consecutive particles of a wavefront then
-	// lie in different cells, and their deposit atomics spread over the LDS
-	// banks instead of piling onto the 4 nodes of one cell.
-	int64_t *tmp = (int64_t *) malloc(sizeof(int64_t) * 4 * (ADJ_CHUNK + 1));
-	if(!tmp)
-	{
-		free(cell);
-		return CPIC_ENOMEM;
-	}
-	int64_t *rs = tmp + (ADJ_CHUNK + 1), *pos = rs + (ADJ_CHUNK + 1), *act = pos + (ADJ_CHUNK + 1);
-	for(int64_t c0 = 0; c0 < n; c0 += ADJ_CHUNK)
-	{
-		int64_t c1 = c0 + ADJ_CHUNK < n ? c0 + ADJ_CHUNK : n, nr = 0, k = 0;
-		for(int64_t j = c0; j < c1; j++)
-			if(j == c0 || cell[a->perm[j]] != cell[a->perm[j - 1]])
-				rs[nr++] = j;
-		rs[nr] = c1;
-		for(int64_t q = 0; q < nr; q++)
-		{
-			act[q] = q;
-			pos[q] = rs[q];
-		}
-		// one particle of every cell that still has one, per round: O(chunk)
-		for(int64_t na = nr; na > 0; )
-		{
-			int64_t w = 0;
-			for(int64_t q = 0; q < na; q++)
-			{
-				int64_t r = act[q];
-				tmp[k++] = a->perm[pos[r]++];
-				if(pos[r] < rs[r + 1])
-					act[w++] = r;
-			}
-			na = w;
-		}
-		memcpy(a->perm + c0, tmp, sizeof(int64_t) * (size_t) (c1 - c0));
-	}
-	free(tmp);
-	free(cell);
-	return 0;
+	// Within each block's chunk (same cells, so the same LDS box) the cells
+	// are taken round-robin: the 64 consecutive particles of a wavefront
+	// then lie in different cells, and their deposit atomics spread over the
+	// LDS banks instead of piling onto the 4 nodes of one cell.
+	return sort_by_cell_rr(a->n, x, y, a->g.idx, a->g.idy, a->g.nx, a->g.ny, ADJ_CHUNK,
+			a->perm) ? CPIC_ENOMEM : 0;
 }
 
 // Device arrays in sorted order -> host arrays in the caller's order.
@@ -851,11 +795,7 @@ download_unsorted(cpic_adj *a, double *const *src, double *const *dst)
 	HIPCHK(hipStreamSynchronize(a->d->stream));
 	for(int c = 0; c < 4; c++)
 		if(dst[c])
-		{
-			const double *h = a->hbuf + c * n;
-			for(int64_t i = 0; i < n; i++)
-				dst[c][a->perm[i]] = h[i];
-		}
+			scatter_perm(n, a->perm, (const double *) (a->hbuf + c * n), dst[c]);
 	return 0;
 }
 
@@ -867,10 +807,18 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	int ret, cur = 0;
 	if(!a || !x || !y || !ux || !uy || nsteps < 1 || checkpoint_every < 1)
 		return CPIC_EINVAL;
-	for(int64_t i = 0; i < a->n; i++)
-		if(!(x[i] >= 0.0 && x[i] <= a->g.L0 && y[i] >= 0.0 && y[i] <= a->g.L1) ||
-				!std::isfinite(ux[i]) || !std::isfinite(uy[i]))
-			return CPIC_EINVAL;
+	std::atomic<int> invalid(0);
+	parallel_ranges(a->n, host_threads(a->n, 1 << 16), [&](int, int64_t lo, int64_t hi) {
+		for(int64_t i = lo; i < hi; i++)
+			if(!(x[i] >= 0.0 && x[i] <= a->g.L0 && y[i] >= 0.0 && y[i] <= a->g.L1) ||
+					!std::isfinite(ux[i]) || !std::isfinite(uy[i]))
+			{
+				invalid = 1;
+				return;
+			}
+	});
+	if(invalid)
+		return CPIC_EINVAL;
 	hipSetDevice(a->d->device);
 	int64_t K = checkpoint_every, nck = (nsteps + K - 1) / K, n = a->n;
 	if((ret = grow(&a->ck, &a->ckcap, 4 * n * nck)) || (ret = grow(&a->hist, &a->histcap, nsteps)))
@@ -883,8 +831,7 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	int32_t *ssp = (int32_t *) malloc(sizeof(int32_t) * (size_t) n);
 	if(!ssp)
 		return CPIC_ENOMEM;
-	for(int64_t i = 0; i < n; i++)
-		ssp[i] = a->hsp[a->perm[i]];
+	gather_perm(n, a->perm, (const int32_t *) a->hsp, ssp);
 	hipError_t e = hipMemcpy(a->sp, ssp, sizeof(int32_t) * n, hipMemcpyHostToDevice);
 	free(ssp);
 	if(e != hipSuccess)
@@ -892,8 +839,7 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	for(int c = 0; c < 4; c++)
 	{
 		double *h = a->hbuf + c * n;
-		for(int64_t i = 0; i < n; i++)
-			h[i] = src[c][a->perm[i]];
+		gather_perm(n, a->perm, src[c], h);
 		HIPCHK(hipMemcpyAsync(dev[c], h, pn, hipMemcpyHostToDevice, a->d->stream));
 	}
 	HIPCHK(hipMemsetAsync(a->derr, 0, sizeof(int), a->d->stream));

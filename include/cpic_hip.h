@@ -121,7 +121,10 @@ int cpic_dev_count(cpic_dev_t *dev, int32_t species, int64_t *n);
 /* sim_pre_step (src/sim.c:209-236): initial rho and E (iter -1). */
 int cpic_dev_pre_step(cpic_dev_t *dev);
 
-/* stage_field_E (src/field.c:451-501): MFT solve rho -> phi, E = -grad phi. */
+/* stage_field_E (src/field.c:451-501): MFT solve rho -> phi, E = -grad phi.
+ * The E array itself is computed when something reads it (a download,
+ * per-particle E output) or before phi is replaced; the push computes the
+ * same values from phi.  An uploaded E is what the next push uses. */
 int cpic_dev_field_E(cpic_dev_t *dev, int64_t iter);
 
 /* stage_plasma_E (src/particle.c:233-248, interpolate_f2p_E

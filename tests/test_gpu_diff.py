@@ -101,6 +101,59 @@ def test_adjoint_gradient_matches_torch_reference_64():
     run.close()
 
 
+def test_adjoint_lds_boxes_across_the_periodic_boundary():
+    """64x64 at 64 particles per cell (262,144 particles): each block's 1,024
+    particles span a quarter row, so every deposit uses an LDS box, and the
+    chunks that end one grid row and begin the next, or hold particles that
+    cross x = 0 (v0 = 0.5), use a box that wraps.  Forward state and field
+    energies vs the oracle at 1e-12; gradients vs torch autograd at 1e-7
+    over 10 steps."""
+    D = diff()
+    from torch_pic import TorchPIC
+    c = c1_state(nx=64, n=131072, v0=0.5, sigma=0.05, seed=37)
+    nsteps = 10
+    run = D.AdjointRun(c["nx"], c["nx"], c["L"], c["dt"], c["q"], c["m"], c["sp"])
+    loss, W = run.forward(*c["st"], nsteps, 5)
+    o = po.Oracle(c["nx"], c["nx"], c["L"], c["dt"], c["q"], c["m"])
+    for s, (ids, x, y, ux, uy) in enumerate(c["parts"]):
+        o.add(s, ids, x, y, ux, uy)
+    o.pre_step()
+    Wo = []
+    for it in range(nsteps):
+        o.stage("field_E")
+        Wo.append(o.field_energy())
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        o.iter = it + 1
+    assert rel_l2(W, Wo) < 1e-12
+    x, y, ux, uy = run.state()
+    n = len(c["parts"][0][0])
+    crossed = 0
+    for s in range(2):
+        a = o.particles(s)
+        sl = slice(s * n, (s + 1) * n)
+        for k, v in (("x", x), ("y", y), ("ux", ux), ("uy", uy)):
+            assert rel_l2(v[sl], a[k]) < 1e-12, (s, k)
+        crossed += int(np.sum(np.abs(x[sl] - c["st"][0][sl]) > c["L"][0] / 2))
+    assert crossed > 1000, crossed
+    r0 = torch.tensor(np.stack(c["st"][:2], 1), requires_grad=True)
+    u0 = torch.tensor(np.stack(c["st"][2:], 1), requires_grad=True)
+    lg = D.field_energy_loss(r0, u0, run, nsteps, 5)
+    lg.backward()
+    ref = TorchPIC(c["nx"], c["nx"], c["L"], c["dt"], c["q"], c["m"])
+    st = [torch.tensor(a, requires_grad=True) for a in c["st"]]
+    lref, _ = ref.run(*st, c["sp"], nsteps)
+    lref.backward()
+    assert abs(float(lg) - float(lref)) < 1e-10 * abs(float(lref))
+    errs = [rel_l2(g.numpy(), gref.numpy())
+            for g, gref in ((r0.grad[:, 0], st[0].grad), (r0.grad[:, 1], st[1].grad),
+                            (u0.grad[:, 0], st[2].grad), (u0.grad[:, 1], st[3].grad))]
+    print("gradient rel L2 vs torch reference (x, y, ux, uy):", errs, flush=True)
+    assert max(errs) < 1e-7, errs
+    run.close()
+
+
 def test_c5_gradients_v0_sigma_vs_finite_differences():
     """256^2, 4,194,304 particles (64 ppc), two-stream, B = 0, 200 steps,
     checkpoint every 20: d loss / d v0 and d loss / d sigma through the
