@@ -351,13 +351,17 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 	int *pb = d->hpre + PRE_BUCKET * (P + 1);
 	for(p = 0; p < P; p++)
 	{
-		Geo *g = &d->hgeo[p];
-		geo_fill(&descs[p], g);
-		if(!cpic_fft_blocking_ok(g))
+		geo_fill(&descs[p], &d->hgeo[p]);
+		if(!cpic_fft_blocking_ok(&d->hgeo[p]))
 		{
 			dev_free(d);
 			return CPIC_EINVAL;
 		}
+	}
+	cpic_fft_spread(d->hgeo, P);
+	for(p = 0; p < P; p++)
+	{
+		Geo *g = &d->hgeo[p];
 		int64_t n = (int64_t) g->nx * g->ny, hx = g->nx / 2 + 1;
 		g->node0 = nodes;
 		g->ghat0 = ghat;

@@ -208,6 +208,7 @@ struct cpic_dev {
 // helpers implemented in the .hip files
 int cpic_fft_init(cpic_dev *d);
 void cpic_fft_blocking(Geo *g);
+void cpic_fft_spread(Geo *g, int P);
 int cpic_fft_blocking_ok(const Geo *g);
 size_t cpic_push_lds(const cpic_dev *d);
 int cpic_set_iter(cpic_dev *d, int64_t iter);
@@ -262,6 +263,7 @@ void cpic_kt_end(cpic_dev *d, int slot);
 // FFT_MAX_VALUES complex values (the LDS budget above keeps every pass
 // within it)
 #define FFT_THREADS 1024
+#define FFT_MIN_BLOCKS 256       // blocks per FFT pass worth spreading to (one per CU)
 #define FFT_MAX_VALUES 8192
 
 static inline int ilog2(int64_t n)

@@ -624,6 +624,45 @@ cpic_fft_blocking(Geo *g)
 	g->C = (int) C;
 }
 
+// Fewer blocks per FFT pass than CUs leave most of the chip idle (a single
+// 256^2 grid has 8 row blocks of 32 rows): halve the rows (columns) per
+// block of the plasma with the most, down to one, until the batch has
+// FFT_MIN_BLOCKS blocks per pass.  Large grids and big batches keep their
+// blocking.
+void
+cpic_fft_spread(Geo *g, int P)
+{
+	for(;;)
+	{
+		int64_t rows = 0;
+		int best = -1;
+		for(int p = 0; p < P; p++)
+		{
+			rows += g[p].ny / g[p].R;
+			if(g[p].R > 1 && (best < 0 || g[p].R > g[best].R))
+				best = p;
+		}
+		if(rows >= FFT_MIN_BLOCKS || best < 0)
+			break;
+		g[best].R /= 2;
+	}
+	for(;;)
+	{
+		int64_t cols = 0;
+		int best = -1;
+		for(int p = 0; p < P; p++)
+		{
+			int64_t hx = g[p].nx / 2 + 1;
+			cols += (hx + g[p].C - 1) / g[p].C;
+			if(g[p].C > 1 && (best < 0 || g[p].C > g[best].C))
+				best = p;
+		}
+		if(cols >= FFT_MIN_BLOCKS || best < 0)
+			break;
+		g[best].C /= 2;
+	}
+}
+
 // Whether the blocking keeps every FFT pass within FFT_MAX_VALUES values and
 // the LDS limit per block.
 int
