@@ -702,13 +702,14 @@ deposit(cpic_adj *a, const double *x, const double *y, double *rho)
 	return 0;
 }
 
-// phi and E from rho; with w, the field energy of rho into *w
+// phi and E from rho; with w, the field energy of rho into *w; with
+// rho_zero, that (other) buffer is zeroed by the solve's last pass
 static int
-fields(cpic_adj *a, const double *rho, double *w)
+fields(cpic_adj *a, const double *rho, double *w, double *rho_zero)
 {
 	int ret;
 	int64_t nodes = (int64_t) a->g.nx * a->g.ny;
-	if((ret = cpic_solve_ptr(a->d, rho, a->phi)))
+	if((ret = cpic_solve_ptr_zero(a->d, rho, a->phi, rho_zero)))
 		return ret;
 	if(w)
 	{
@@ -727,9 +728,7 @@ forward_iter(cpic_adj *a, int64_t it, int *cur, double *w)
 {
 	int ret;
 	int c = *cur;
-	if((ret = fields(a, a->rho[c], w)))
-		return ret;
-	if((ret = zero(a, a->rho[1 - c], (int64_t) a->g.nx * a->g.ny)))
+	if((ret = fields(a, a->rho[c], w, a->rho[1 - c])))
 		return ret;
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
@@ -968,7 +967,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			const double *sx = a->seg + 2 * n * (it - it0), *sy = sx + n;
 			int first = it == 0;
 			double dtp = first ? -a->g.dt / 2 : a->g.dt;
-			if((ret = deposit(a, sx, sy, a->rho[0])) || (ret = fields(a, a->rho[0], NULL)) ||
+			if((ret = deposit(a, sx, sy, a->rho[0])) || (ret = fields(a, a->rho[0], NULL, NULL)) ||
 					(ret = zero(a, (double *) a->lE, 2 * nodes)))
 				break;
 			hipLaunchKernelGGL(k_adj_rev_a, dim3((int) ((n + REV_CHUNK - 1) / REV_CHUNK)),

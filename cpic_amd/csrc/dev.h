@@ -215,6 +215,9 @@ int cpic_set_iter(cpic_dev *d, int64_t iter);
 int cpic_solve(cpic_dev *d, int want_energy);
 int cpic_solve_step(cpic_dev *d);
 int cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi);
+// the same, also zeroing the nodes of rho_zero (another buffer) in the
+// inverse row pass
+int cpic_solve_ptr_zero(cpic_dev *d, const double *rho, double *phi, double *rho_zero);
 int cpic_efield_ptr(cpic_dev *d, const double *phi, void *E);
 int cpic_efield(cpic_dev *d);
 int cpic_parts_init(cpic_dev *d);

@@ -741,6 +741,12 @@ cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi)
 }
 
 int
+cpic_solve_ptr_zero(cpic_dev *d, const double *rho, double *phi, double *rho_zero)
+{
+	return solve(d, rho, phi, (DiagSlot *) NULL, rho_zero, (DiagSlot *) NULL);
+}
+
+int
 cpic_efield_ptr(cpic_dev *d, const double *phi, void *E)
 {
 	const int P = d->P;
