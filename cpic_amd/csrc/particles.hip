@@ -47,6 +47,7 @@ template <> struct V2<double> { typedef double2 t; };
 template <> struct V2<float> { typedef float2 t; };
 // the same pairs as clang vectors, which load through address-space pointers
 template <typename T> struct N2 { typedef T t __attribute__((ext_vector_type(2))); };
+template <typename T, int U> struct NU { typedef T t __attribute__((ext_vector_type(U))); };
 
 // Arguments of the tile kernels.  Block b handles batch tile order[b], finds
 // the tile's plasma in pre_tile and takes the plasma's geometry and
@@ -532,12 +533,36 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 }
 
 // Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
-// thread tid takes base + u*CPIC_BLOCK + tid (coalesced loads).
+// thread tid takes base + U*tid + u with vector loads when the chunk lies in
+// the bucket's first segment, else base + u*CPIC_BLOCK + tid (coalesced
+// loads either way; every slot of the chunk exactly once).
 template <typename T, bool IDS, int U>
 static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
 		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
 {
 	const GLB_AS T *pb = bucket_base<T>(p, bstart);
+	if(base + U * CPIC_BLOCK <= n1)
+	{
+		// the whole chunk lies in the first segment (the usual case): thread
+		// tid takes the U consecutive slots base + U*tid ... (U divides the
+		// group of 64, so they are contiguous), one vector load per component
+		typedef typename NU<T, U>::t VU;
+		const int j0 = base + U * tid;
+		const GLB_AS VU *r = (const GLB_AS VU *) (pb + pslot_rel(j0));
+		VU vx = r[0], vy = r[PGROUP / U], vux = r[2 * PGROUP / U], vuy = r[3 * PGROUP / U];
+#pragma unroll
+		for(int u = 0; u < U; u++)
+		{
+			act[u] = true;
+			q[u].x = vx[u];
+			q[u].y = vy[u];
+			q[u].ux = vux[u];
+			q[u].uy = vuy[u];
+			if(IDS)
+				q[u].id = gptr<int64_t>(p.id)[bstart + j0 + u];
+		}
+		return;
+	}
 #pragma unroll
 	for(int u = 0; u < U; u++)
 	{
