@@ -533,19 +533,21 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 }
 
 // Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
-// thread tid takes base + U*tid + u with vector loads when the chunk lies in
-// the bucket's first segment, else base + u*CPIC_BLOCK + tid (coalesced
-// loads either way; every slot of the chunk exactly once).
+// thread tid takes base + U*tid + u with vector loads when U = 4 and the
+// chunk lies in the bucket's first segment, else base + u*CPIC_BLOCK + tid
+// (coalesced loads either way; every slot of the chunk exactly once).
 template <typename T, bool IDS, int U>
 static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
 		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
 {
 	const GLB_AS T *pb = bucket_base<T>(p, bstart);
-	if(base + U * CPIC_BLOCK <= n1)
+	if(U == 4 && base + U * CPIC_BLOCK <= n1)
 	{
 		// the whole chunk lies in the first segment (the usual case): thread
 		// tid takes the U consecutive slots base + U*tid ... (U divides the
-		// group of 64, so they are contiguous), one vector load per component
+		// group of 64, so they are contiguous), one vector load per component.
+		// Measured with U = 4 only: with U = 2 (two-element vectors) the push
+		// was 4 % slower than with the per-slot loads below.
 		typedef typename NU<T, U>::t VU;
 		const int j0 = base + U * tid;
 		const GLB_AS VU *r = (const GLB_AS VU *) (pb + pslot_rel(j0));
