@@ -1,7 +1,7 @@
 /*
  * cpic_oracle.c -- FP64 CPU restatement of cpic's per-timestep PIC loop.
  *
- * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, by
+ * TEST INFRASTRUCTURE ONLY (it checks the GPU results): loaded by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  Never part of
  * the shipped path.  Built with -ffp-contract=off; see cpic_oracle.h for the
  * explicit fma() policy.

@@ -1,6 +1,6 @@
 /*
  * cpic_oracle.h -- FP64 CPU restatement of cpic's per-timestep 2D electrostatic
- * PIC loop (rodarima/cpic), used ONLY as the checker.
+ * PIC loop (rodarima/cpic), used ONLY to check the GPU results.
  *
  * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library.  The product

@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
-by bench.py's cpu_baseline leg, always as the checker or the CPU baseline,
+by bench.py's cpu_baseline leg, always to check results or as the CPU baseline,
 never as the measured product path.
 """
 import ctypes as C

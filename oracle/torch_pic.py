@@ -1,6 +1,6 @@
 """FP64 torch restatement of the PIC loop, differentiated by torch autograd.
 
-Test infrastructure only (the checker of the library's adjoint kernels):
+Test infrastructure only (it checks the library's adjoint kernels):
 it follows the same discrete model as the C oracle -- CIC weights of
 relative_position_grid (src/interpolate.c:39-66), the MFT Green's function
 (src/solver.c:269-280) applied with torch.fft, the central-difference E
