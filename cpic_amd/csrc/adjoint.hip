@@ -57,6 +57,8 @@ struct cpic_adj {
 	int64_t ckcap;
 	double *seg;              // positions of one segment's iterations, 2n doubles each
 	int64_t segcap;
+	double *segf;             // rho and E of one segment's iterations, 3 nodes doubles each
+	int64_t segfcap;
 	double *hist;             // W per iteration
 	int64_t histcap;
 	double *wk;               // recompute state of the reverse sweep, 4n doubles
@@ -561,7 +563,8 @@ adj_free(cpic_adj *a)
 	hipFree(a->rho[0]); hipFree(a->rho[1]); hipFree(a->phi); hipFree(a->psi); hipFree(a->lrho);
 	hipFree(a->E); hipFree(a->lE);
 	hipFree(a->lx); hipFree(a->ly); hipFree(a->lux); hipFree(a->luy);
-	hipFree(a->ck); hipFree(a->seg); hipFree(a->hist); hipFree(a->part); hipFree(a->derr);
+	hipFree(a->ck); hipFree(a->seg); hipFree(a->segf); hipFree(a->hist); hipFree(a->part);
+	hipFree(a->derr);
 	hipFree(a->wk);
 	if(a->evok)
 		for(int i = 0; i < 4; i++)
@@ -702,10 +705,10 @@ deposit(cpic_adj *a, const double *x, const double *y, double *rho)
 	return 0;
 }
 
-// phi and E from rho; with w, the field energy of rho into *w; with
-// rho_zero, that (other) buffer is zeroed by the solve's last pass
+// phi and E (into E) from rho; with w, the field energy of rho into *w;
+// with rho_zero, that (other) buffer is zeroed by the solve's last pass
 static int
-fields(cpic_adj *a, const double *rho, double *w, double *rho_zero)
+fields(cpic_adj *a, const double *rho, double *w, double *rho_zero, double2 *E)
 {
 	int ret;
 	int64_t nodes = (int64_t) a->g.nx * a->g.ny;
@@ -719,23 +722,35 @@ fields(cpic_adj *a, const double *rho, double *w, double *rho_zero)
 				-0.5 * a->g.e0, w);
 		HIPCHK(hipGetLastError());
 	}
-	return cpic_efield_ptr(a->d, a->phi, a->E);
+	return cpic_efield_ptr(a->d, a->phi, E);
 }
 
-// one forward iteration from the state in (x, y, ux, uy) and rho[cur] = Dep(r)
+// one forward iteration from the state in (x, y, ux, uy) and rho_in =
+// Dep(r): E of this iteration into E, Dep(r') of the moved particles into
+// rho_out
 static int
-forward_iter(cpic_adj *a, int64_t it, int *cur, double *w)
+forward_step(cpic_adj *a, int64_t it, const double *rho_in, double *rho_out, double2 *E,
+		double *w)
 {
 	int ret;
-	int c = *cur;
-	if((ret = fields(a, a->rho[c], w, a->rho[1 - c])))
+	if((ret = fields(a, rho_in, w, rho_out, E)))
 		return ret;
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
 	hipLaunchKernelGGL(k_adj_push, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
 			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, a->x, a->y, a->ux,
-			a->uy, (const double2 *) a->E, a->rho[1 - c], a->derr);
+			a->uy, (const double2 *) E, rho_out, a->derr);
 	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// forward_step on the double-buffered rho[cur] / rho[1-cur] and the E array
+static int
+forward_iter(cpic_adj *a, int64_t it, int *cur, double *w)
+{
+	int ret, c = *cur;
+	if((ret = forward_step(a, it, a->rho[c], a->rho[1 - c], a->E, w)))
+		return ret;
 	*cur = 1 - c;
 	return 0;
 }
@@ -922,7 +937,8 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	hipSetDevice(a->d->device);
 	const int64_t n = a->n, K = a->K, T = a->nsteps, nodes = (int64_t) a->g.nx * a->g.ny;
 	const int64_t nck = (T + K - 1) / K;
-	if((ret = grow(&a->seg, &a->segcap, 2 * n * K)))
+	if((ret = grow(&a->seg, &a->segcap, 2 * n * K)) ||
+			(ret = grow(&a->segf, &a->segfcap, 3 * nodes * K)))
 		return ret;
 	if(!a->wk && hipMalloc(&a->wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
 	{
@@ -942,14 +958,18 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	a->ux = wk + 2 * n;
 	a->uy = wk + 3 * n;
 	ret = 0;
+	// the segment's rho and E of iteration i (recomputed below, read by the
+	// reverse iteration): no deposit or solve of the forward's fields again
+	double *const segf = a->segf;
+	auto seg_rho = [segf, nodes](int64_t i) { return segf + 3 * nodes * i; };
+	auto seg_E = [segf, nodes](int64_t i) { return (double2 *) (segf + 3 * nodes * i + nodes); };
 	for(int64_t sg = nck - 1; sg >= 0 && !ret; sg--)
 	{
 		int64_t it0 = sg * K, it1 = it0 + K < T ? it0 + K : T;
-		int cur = 0;
 		const double *c = a->ck + 4 * n * sg;
 		if((ret = copy_state(a, wk, c, c + n, c + 2 * n, c + 3 * n, hipMemcpyDeviceToDevice)))
 			break;
-		if((ret = deposit(a, a->x, a->y, a->rho[0])))
+		if((ret = deposit(a, a->x, a->y, seg_rho(0))))
 			break;
 		for(int64_t it = it0; it < it1 && !ret; it++)
 		{
@@ -960,23 +980,26 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 						a->d->stream) != hipSuccess)
 				ret = CPIC_EHIP;
 			else if(it + 1 < it1)
-				ret = forward_iter(a, it, &cur, (double *) NULL);
+				ret = forward_step(a, it, seg_rho(it - it0), seg_rho(it + 1 - it0),
+						seg_E(it - it0), (double *) NULL);
+			else
+				ret = fields(a, seg_rho(it - it0), (double *) NULL, (double *) NULL,
+						seg_E(it - it0));
 		}
 		for(int64_t it = it1 - 1; it >= it0 && !ret; it--)
 		{
 			const double *sx = a->seg + 2 * n * (it - it0), *sy = sx + n;
 			int first = it == 0;
 			double dtp = first ? -a->g.dt / 2 : a->g.dt;
-			if((ret = deposit(a, sx, sy, a->rho[0])) || (ret = fields(a, a->rho[0], NULL, NULL)) ||
-					(ret = zero(a, (double *) a->lE, 2 * nodes)))
+			if((ret = zero(a, (double *) a->lE, 2 * nodes)))
 				break;
 			hipLaunchKernelGGL(k_adj_rev_a, dim3((int) ((n + REV_CHUNK - 1) / REV_CHUNK)),
 					dim3(CPIC_BLOCK), 0, a->d->stream,
 					a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
-					(const double2 *) a->E, a->lx, a->ly, a->lux, a->luy, a->lE);
+					(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, a->lE);
 			hipLaunchKernelGGL(k_adj_field, dim3(pblocks(nodes)), dim3(CPIC_BLOCK), 0,
-					a->d->stream, a->g, (const double2 *) a->lE, a->rho[0], dloss * a->g.e0,
-					a->psi);
+					a->d->stream, a->g, (const double2 *) a->lE, seg_rho(it - it0),
+					dloss * a->g.e0, a->psi);
 			if(hipGetLastError() != hipSuccess)
 			{
 				ret = CPIC_EHIP;
