@@ -286,12 +286,14 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 	box_flush(g, b, sacc, 1, rho, 1);
 }
 
-// gather + Boris + move + wrap, then deposit at the new position into rho_next
+// gather + Boris + move + wrap, then deposit at the new position into
+// rho_next; positions are read from (x, y) and written to (xo, yo), which may
+// be the same arrays
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
-		const int32_t *__restrict__ sp, double *__restrict__ x, double *__restrict__ y,
-		double *__restrict__ ux, double *__restrict__ uy, const double2 *__restrict__ E,
-		double *__restrict__ rho_next, int *derr)
+		const int32_t *__restrict__ sp, const double *x, const double *y, double *xo,
+		double *yo, double *__restrict__ ux, double *__restrict__ uy,
+		const double2 *__restrict__ E, double *__restrict__ rho_next, int *derr)
 {
 	__shared__ double sacc[ADJ_BOX_NODES];
 	__shared__ int s6[6];
@@ -339,8 +341,8 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 			nan_ = true;
 			px = py = 0.0;
 		}
-		x[i] = px;
-		y[i] = py;
+		xo[i] = px;
+		yo[i] = py;
 		ux[i] = uu;
 		uy[i] = vv;
 		cell_of(g, px, py, ix[u], iy[u], rx[u], ry[u]);
@@ -725,12 +727,12 @@ fields(cpic_adj *a, const double *rho, double *w, double *rho_zero, double2 *E)
 	return cpic_efield_ptr(a->d, a->phi, E);
 }
 
-// one forward iteration from the state in (x, y, ux, uy) and rho_in =
-// Dep(r): E of this iteration into E, Dep(r') of the moved particles into
-// rho_out
+// one forward iteration from positions (x, y), velocities (ux, uy) and
+// rho_in = Dep(r): E of this iteration into E, the moved positions into
+// (xo, yo) (may be (x, y)), Dep(r') of the moved particles into rho_out
 static int
-forward_step(cpic_adj *a, int64_t it, const double *rho_in, double *rho_out, double2 *E,
-		double *w)
+forward_step(cpic_adj *a, int64_t it, const double *x, const double *y, double *xo,
+		double *yo, const double *rho_in, double *rho_out, double2 *E, double *w)
 {
 	int ret;
 	if((ret = fields(a, rho_in, w, rho_out, E)))
@@ -738,18 +740,19 @@ forward_step(cpic_adj *a, int64_t it, const double *rho_in, double *rho_out, dou
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
 	hipLaunchKernelGGL(k_adj_push, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
-			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, a->x, a->y, a->ux,
+			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, x, y, xo, yo, a->ux,
 			a->uy, (const double2 *) E, rho_out, a->derr);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
 
-// forward_step on the double-buffered rho[cur] / rho[1-cur] and the E array
+// forward_step in place on the state, on the double-buffered rho[cur] /
+// rho[1-cur] and the E array
 static int
 forward_iter(cpic_adj *a, int64_t it, int *cur, double *w)
 {
 	int ret, c = *cur;
-	if((ret = forward_step(a, it, a->rho[c], a->rho[1 - c], a->E, w)))
+	if((ret = forward_step(a, it, a->x, a->y, a->x, a->y, a->rho[c], a->rho[1 - c], a->E, w)))
 		return ret;
 	*cur = 1 - c;
 	return 0;
@@ -966,22 +969,28 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	for(int64_t sg = nck - 1; sg >= 0 && !ret; sg--)
 	{
 		int64_t it0 = sg * K, it1 = it0 + K < T ? it0 + K : T;
+		// positions of iteration i live in the segment's slot i: the
+		// checkpoint's go to slot 0, each push writes the next slot
 		const double *c = a->ck + 4 * n * sg;
-		if((ret = copy_state(a, wk, c, c + n, c + 2 * n, c + 3 * n, hipMemcpyDeviceToDevice)))
+		if((ret = copy_state(a, a->seg, c, c + n, (const double *) NULL, (const double *) NULL,
+				hipMemcpyDeviceToDevice)))
 			break;
-		if((ret = deposit(a, a->x, a->y, seg_rho(0))))
+		if(hipMemcpyAsync(a->ux, c + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+					a->d->stream) != hipSuccess ||
+				hipMemcpyAsync(a->uy, c + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+					a->d->stream) != hipSuccess)
+		{
+			ret = CPIC_EHIP;
+			break;
+		}
+		if((ret = deposit(a, a->seg, a->seg + n, seg_rho(0))))
 			break;
 		for(int64_t it = it0; it < it1 && !ret; it++)
 		{
 			double *sx = a->seg + 2 * n * (it - it0);
-			if(hipMemcpyAsync(sx, a->x, sizeof(double) * n, hipMemcpyDeviceToDevice,
-						a->d->stream) != hipSuccess ||
-					hipMemcpyAsync(sx + n, a->y, sizeof(double) * n, hipMemcpyDeviceToDevice,
-						a->d->stream) != hipSuccess)
-				ret = CPIC_EHIP;
-			else if(it + 1 < it1)
-				ret = forward_step(a, it, seg_rho(it - it0), seg_rho(it + 1 - it0),
-						seg_E(it - it0), (double *) NULL);
+			if(it + 1 < it1)
+				ret = forward_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n, seg_rho(it - it0),
+						seg_rho(it + 1 - it0), seg_E(it - it0), (double *) NULL);
 			else
 				ret = fields(a, seg_rho(it - it0), (double *) NULL, (double *) NULL,
 						seg_E(it - it0));
