@@ -940,8 +940,10 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	hipSetDevice(a->d->device);
 	const int64_t n = a->n, K = a->K, T = a->nsteps, nodes = (int64_t) a->g.nx * a->g.ny;
 	const int64_t nck = (T + K - 1) / K;
-	if((ret = grow(&a->seg, &a->segcap, 2 * n * K)) ||
-			(ret = grow(&a->segf, &a->segfcap, 3 * nodes * K)))
+	// a segment holds at most min(K, T) iterations (the last one T - (nck-1) K)
+	const int64_t Kseg = K < T ? K : T;
+	if((ret = grow(&a->seg, &a->segcap, 2 * n * Kseg)) ||
+			(ret = grow(&a->segf, &a->segfcap, 3 * nodes * Kseg)))
 		return ret;
 	if(!a->wk && hipMalloc(&a->wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
 	{

@@ -112,6 +112,13 @@ kt_collect(cpic_dev *d)
 
 // -------------------------------------------------------------- setup ---
 
+// After a re-sort that failed part way the particle storage is gone (or a
+// buffer the push writes was freed): the handle refuses further work.
+#define BROKEN_CHECK(d) do { \
+		if((d) && (d)->broken) \
+			return CPIC_ESTATE; \
+	} while(0)
+
 // Makes the handle's device current (not while a step is being captured:
 // the stream is then recording and the device is already current).
 static void
@@ -507,6 +514,7 @@ extern "C" int
 cpic_batch_upload_particles(cpic_dev_t *d, int32_t p, int32_t s, int64_t n, const int64_t *id,
 		const double *x, const double *y, const double *ux, const double *uy)
 {
+	BROKEN_CHECK(d);
 	if(!plasma_ok(d, p) || s < 0 || s >= d->hgeo[p].ns || n < 0 ||
 			(n > 0 && (!x || !y || !ux || !uy)))
 		return CPIC_EINVAL;
@@ -637,6 +645,7 @@ derr_check(cpic_dev *d)
 extern "C" int
 cpic_dev_sync(cpic_dev_t *d)
 {
+	BROKEN_CHECK(d);
 	if(!d)
 		return CPIC_EINVAL;
 	set_device(d);
@@ -668,6 +677,7 @@ cpic_dev_count(cpic_dev_t *d, int32_t s, int64_t *n)
 extern "C" int
 cpic_batch_download_particles(cpic_dev_t *d, int32_t p, int32_t s, cpic_particles_t *out)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!plasma_ok(d, p) || !out || s < 0 || s >= d->hgeo[p].ns)
 		return CPIC_EINVAL;
@@ -692,6 +702,7 @@ cpic_dev_download_particles(cpic_dev_t *d, int32_t s, cpic_particles_t *out)
 extern "C" int
 cpic_batch_cell_counts(cpic_dev_t *d, int32_t p, int32_t s, int64_t *counts)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!plasma_ok(d, p) || !counts || s < 0 || s >= d->hgeo[p].ns)
 		return CPIC_EINVAL;
@@ -710,6 +721,7 @@ cpic_dev_cell_counts(cpic_dev_t *d, int32_t s, int64_t *counts)
 extern "C" int
 cpic_dev_rebuild(cpic_dev_t *d)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
@@ -754,6 +766,7 @@ ensure_E(cpic_dev *d)
 extern "C" int
 cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
@@ -771,6 +784,7 @@ cpic_dev_field_E(cpic_dev_t *d, int64_t iter)
 extern "C" int
 cpic_dev_solve_xy(cpic_dev_t *d)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
@@ -785,6 +799,7 @@ cpic_dev_solve_xy(cpic_dev_t *d)
 extern "C" int
 cpic_dev_plasma_E(cpic_dev_t *d, int64_t iter)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	(void) iter;
 	if(!d)
@@ -809,6 +824,7 @@ zero_rho(cpic_dev *d, double *r)
 extern "C" int
 cpic_dev_plasma_r(cpic_dev_t *d, int64_t iter)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
@@ -835,6 +851,7 @@ cpic_dev_plasma_r(cpic_dev_t *d, int64_t iter)
 extern "C" int
 cpic_dev_field_rho(cpic_dev_t *d, int64_t iter)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	(void) iter;
 	if(!d)
@@ -858,6 +875,7 @@ cpic_dev_field_rho(cpic_dev_t *d, int64_t iter)
 extern "C" int
 cpic_dev_pre_step(cpic_dev_t *d)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d)
 		return CPIC_EINVAL;
@@ -869,6 +887,7 @@ cpic_dev_pre_step(cpic_dev_t *d)
 extern "C" int
 cpic_dev_step(cpic_dev_t *d, int64_t iter)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if((ret = cpic_dev_field_E(d, iter)))
 		return ret;
@@ -934,6 +953,7 @@ graph_steps(cpic_dev *d, int64_t iter)
 extern "C" int
 cpic_dev_run(cpic_dev_t *d, int64_t iter0, int64_t nsteps)
 {
+	BROKEN_CHECK(d);
 	int ret;
 	if(!d || nsteps < 0)
 		return CPIC_EINVAL;
@@ -978,6 +998,7 @@ cpic_solver_rho_size(int64_t nx, int64_t ny, int64_t *cnx, int64_t *cny)
 extern "C" int
 cpic_batch_download_field(cpic_dev_t *d, int32_t p, int32_t which, double *out)
 {
+	BROKEN_CHECK(d);
 	int64_t n, o;
 	int ret;
 	if(!plasma_ok(d, p) || !out)
@@ -1039,6 +1060,7 @@ cpic_dev_download_field(cpic_dev_t *d, int32_t which, double *out)
 extern "C" int
 cpic_batch_upload_field(cpic_dev_t *d, int32_t p, int32_t which, const double *in)
 {
+	BROKEN_CHECK(d);
 	int64_t n, o;
 	if(!plasma_ok(d, p) || !in)
 		return CPIC_EINVAL;
@@ -1102,6 +1124,7 @@ extern "C" int
 cpic_batch_energy(cpic_dev_t *d, int32_t p, int64_t max, double *field, double *kinetic,
 		double *px, double *py, int64_t *n)
 {
+	BROKEN_CHECK(d);
 	int64_t steps, k;
 	DiagSlot *h;
 	if(!plasma_ok(d, p) || max < 0)
@@ -1164,6 +1187,7 @@ extern "C" int
 cpic_dev_migration(cpic_dev_t *d, int64_t *n, int64_t *id, int32_t *species,
 		int32_t *src_tile, int32_t *dst_tile)
 {
+	BROKEN_CHECK(d);
 	int32_t cnt = 0;
 	int64_t k;
 	if(!d || !n)

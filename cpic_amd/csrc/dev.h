@@ -185,6 +185,8 @@ struct cpic_dev {
 	int64_t iter;            // host mirror of the step counter (next step)
 	int64_t last_iter;       // last iteration pushed
 	int64_t nrebuilds;
+	int broken;              // a re-sort failed part way (its error code):
+	                         // every later call but destroy returns CPIC_ESTATE
 	unsigned long long misplaced_seen;   // ctr->misplaced already acted on
 	unsigned long long fallbacks_seen;   // ctr->fallbacks already acted on
 

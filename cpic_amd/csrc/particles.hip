@@ -1404,6 +1404,17 @@ cpic_parts_free(cpic_dev *d, int which)
 	d->pcap[which] = 0;
 }
 
+// Fault injection for the test of a failed re-sort: while the environment
+// variable CPIC_FAIL_PARTS_ALLOC is "1", particle-buffer allocations fail as
+// if the device were out of memory (after the old buffer is freed, as a real
+// failure would leave it).
+static int
+parts_alloc_fails(void)
+{
+	const char *e = getenv("CPIC_FAIL_PARTS_ALLOC");
+	return e && e[0] == '1' && e[1] == 0;
+}
+
 int
 cpic_parts_alloc(cpic_dev *d, int which, int64_t cap)
 {
@@ -1411,7 +1422,7 @@ cpic_parts_alloc(cpic_dev *d, int which, int64_t cap)
 	size_t e = elem(d);
 	cap = (cap < PGROUP ? PGROUP : cap + PGROUP - 1) / PGROUP * PGROUP;
 	cpic_parts_free(d, which);
-	if(hipMalloc(&p->r, 4 * e * cap) != hipSuccess)
+	if(parts_alloc_fails() || hipMalloc(&p->r, 4 * e * cap) != hipSuccess)
 		goto fail;
 	if((d->flags & CPIC_F_IDS) && hipMalloc(&p->id, sizeof(int64_t) * cap) != hipSuccess)
 		goto fail;
@@ -1580,6 +1591,18 @@ cpic_parts_order(cpic_dev *d, const int *cnt)
 // Full re-sort of the batch: counts per destination bucket (current layout
 // + pending uploads), fresh capacities, scatter into the spare particle
 // arrays.  Pending uploads replace their (plasma, species) wholesale.
+// In the re-sort, a failing HIP call is named on stderr and leaves through
+// the cleanup (which frees the temporaries and marks a half-done re-sort).
+#define RB_CHK(call) do { \
+		hipError_t e_ = (call); \
+		if(e_ != hipSuccess) { \
+			fprintf(stderr, "cpic_hip: %s failed: %s (%s:%d)\n", #call, \
+					hipGetErrorString(e_), __FILE__, __LINE__); \
+			ret = CPIC_EHIP; \
+			goto out; \
+		} \
+	} while(0)
+
 int
 cpic_parts_rebuild(cpic_dev *d)
 {
@@ -1591,11 +1614,14 @@ cpic_parts_rebuild(cpic_dev *d)
 	int64_t *sid = NULL, *dsegpre = NULL;
 	int *dseg = NULL;
 	int ret = CPIC_ENOMEM, nseg = 0;
+	int touched = 0;         // the particle buffers have been changed
 	int *skip = (int *) calloc((size_t) P, sizeof(int));
 	int64_t *segpre = (int64_t *) malloc(sizeof(int64_t) * (size_t) (P * MS + 1));
 	int *seg = (int *) malloc(sizeof(int) * 2 * (size_t) (P * MS));
 	Staged u;
 
+	// captured steps name particle arrays that this may free
+	d->graph_ok = 0;
 	if(!skip || !segpre || !seg)
 		goto out;
 	segpre[0] = 0;
@@ -1613,7 +1639,7 @@ cpic_parts_rebuild(cpic_dev *d)
 			nseg++;
 		}
 
-	HIPCHK(hipStreamSynchronize(d->stream));
+	RB_CHK(hipStreamSynchronize(d->stream));
 	if(hipMalloc(&cnt, sizeof(int) * nb) != hipSuccess ||
 			hipMalloc(&fill, sizeof(int) * nb) != hipSuccess ||
 			hipMalloc(&ncap, sizeof(int) * nb) != hipSuccess ||
@@ -1621,9 +1647,9 @@ cpic_parts_rebuild(cpic_dev *d)
 			hipMalloc(&total, sizeof(int64_t)) != hipSuccess ||
 			hipMalloc(&dskip, sizeof(int) * P) != hipSuccess)
 		goto out;
-	HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int) * nb, d->stream));
-	HIPCHK(hipMemsetAsync(fill, 0, sizeof(int) * nb, d->stream));
-	HIPCHK(hipMemcpyAsync(dskip, skip, sizeof(int) * P, hipMemcpyHostToDevice, d->stream));
+	RB_CHK(hipMemsetAsync(cnt, 0, sizeof(int) * nb, d->stream));
+	RB_CHK(hipMemsetAsync(fill, 0, sizeof(int) * nb, d->stream));
+	RB_CHK(hipMemcpyAsync(dskip, skip, sizeof(int) * P, hipMemcpyHostToDevice, d->stream));
 
 	if(d->pcap[d->pcur] > 0)
 	{
@@ -1644,19 +1670,19 @@ cpic_parts_rebuild(cpic_dev *d)
 			goto out;
 		if((d->flags & CPIC_F_IDS) && hipMalloc(&sid, sizeof(int64_t) * n) != hipSuccess)
 			goto out;
-		HIPCHK(hipMemcpyAsync(dsegpre, segpre, sizeof(int64_t) * (nseg + 1),
+		RB_CHK(hipMemcpyAsync(dsegpre, segpre, sizeof(int64_t) * (nseg + 1),
 				hipMemcpyHostToDevice, d->stream));
-		HIPCHK(hipMemcpyAsync(dseg, seg, sizeof(int) * nseg, hipMemcpyHostToDevice, d->stream));
-		HIPCHK(hipMemcpyAsync(dseg + nseg, seg + P * MS, sizeof(int) * nseg,
+		RB_CHK(hipMemcpyAsync(dseg, seg, sizeof(int) * nseg, hipMemcpyHostToDevice, d->stream));
+		RB_CHK(hipMemcpyAsync(dseg + nseg, seg + P * MS, sizeof(int) * nseg,
 				hipMemcpyHostToDevice, d->stream));
 		for(k = 0; k < nseg; k++)
 		{
 			int i = seg[k] * MS + seg[P * MS + k];
 			int64_t nk = segpre[k + 1] - segpre[k];
-			HIPCHK(hipMemcpyAsync(stage + 4 * segpre[k], d->up_buf[i], sizeof(double) * 4 * nk,
+			RB_CHK(hipMemcpyAsync(stage + 4 * segpre[k], d->up_buf[i], sizeof(double) * 4 * nk,
 					hipMemcpyHostToDevice, d->stream));
 			if(sid && d->up_id[i])
-				HIPCHK(hipMemcpyAsync(sid + segpre[k], d->up_id[i], sizeof(int64_t) * nk,
+				RB_CHK(hipMemcpyAsync(sid + segpre[k], d->up_id[i], sizeof(int64_t) * nk,
 						hipMemcpyHostToDevice, d->stream));
 		}
 		u.segpre = dsegpre;
@@ -1684,11 +1710,12 @@ cpic_parts_rebuild(cpic_dev *d)
 		// capacities in whole groups: bucket starts are multiples of PGROUP
 		hipLaunchKernelGGL(k_scan_caps, dim3(1), dim3(1024), 0, d->stream, cnt, nb, slack, pad,
 				PGROUP, nstart, ncap, total);
-		HIPCHK(hipGetLastError());
-		HIPCHK(hipMemcpyAsync(&tot, total, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
-		HIPCHK(hipStreamSynchronize(d->stream));
+		RB_CHK(hipGetLastError());
+		RB_CHK(hipMemcpyAsync(&tot, total, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
+		RB_CHK(hipStreamSynchronize(d->stream));
 
 		int spare = 1 - d->pcur;
+		touched = 1;
 		if(d->pcap[spare] < tot)
 			if((ret = cpic_parts_alloc(d, spare, tot + tot / 8)))
 				goto out;
@@ -1703,14 +1730,14 @@ cpic_parts_rebuild(cpic_dev *d)
 			DISPATCH_P(k_scatter_stage, dim3(blocks), dim3(CPIC_BLOCK), 0, d->stream,
 					d->geo, u, nstart, fill, d->p[spare]);
 		}
-		HIPCHK(hipGetLastError());
+		RB_CHK(hipGetLastError());
 		// new metadata: n1 = count, res = n2 = 0
-		HIPCHK(hipMemcpyAsync(d->bk.start, nstart, sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, d->stream));
-		HIPCHK(hipMemcpyAsync(d->bk.cap, ncap, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
-		HIPCHK(hipMemcpyAsync(d->bk.n1, cnt, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
-		HIPCHK(hipMemsetAsync(d->bk.res, 0, sizeof(int) * nb, d->stream));
-		HIPCHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
-		HIPCHK(hipStreamSynchronize(d->stream));
+		RB_CHK(hipMemcpyAsync(d->bk.start, nstart, sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, d->stream));
+		RB_CHK(hipMemcpyAsync(d->bk.cap, ncap, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
+		RB_CHK(hipMemcpyAsync(d->bk.n1, cnt, sizeof(int) * nb, hipMemcpyDeviceToDevice, d->stream));
+		RB_CHK(hipMemsetAsync(d->bk.res, 0, sizeof(int) * nb, d->stream));
+		RB_CHK(hipMemsetAsync(d->bk.n2, 0, sizeof(int) * nb, d->stream));
+		RB_CHK(hipStreamSynchronize(d->stream));
 		d->pcur = spare;
 		// the push writes the other buffer: it needs the same capacity
 		if(d->pcap[1 - spare] < d->pcap[spare])
@@ -1719,7 +1746,10 @@ cpic_parts_rebuild(cpic_dev *d)
 		// block order for the new counts
 		int *hc = (int *) malloc(sizeof(int) * (size_t) nb);
 		if(!hc)
+		{
+			ret = CPIC_ENOMEM;
 			goto out;
+		}
 		if(hipMemcpy(hc, d->bk.n1, sizeof(int) * nb, hipMemcpyDeviceToHost) != hipSuccess)
 		{
 			free(hc);
@@ -1748,6 +1778,13 @@ cpic_parts_rebuild(cpic_dev *d)
 	d->graph_ok = 0;         // captured steps name the old particle arrays
 	ret = 0;
 out:
+	if(ret && touched && !d->broken)
+	{
+		// the storage is half rewritten, or a buffer the push writes is gone
+		fprintf(stderr, "cpic_hip: particle re-sort failed (%s); the handle is unusable\n",
+				ret == CPIC_ENOMEM ? "out of device memory" : "HIP error");
+		d->broken = ret;
+	}
 	hipFree(cnt); hipFree(fill); hipFree(ncap); hipFree(nstart); hipFree(total);
 	hipFree(dskip); hipFree(stage); hipFree(sid); hipFree(dsegpre); hipFree(dseg);
 	free(skip); free(segpre); free(seg);

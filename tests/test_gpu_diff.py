@@ -98,6 +98,15 @@ def test_adjoint_gradient_matches_torch_reference_64():
     g200 = run.backward(1.0)
     for a, b in zip(g1, g200):
         assert rel_l2(a, b) < 1e-9
+    # a partial last segment (200 = 6 * 30 + 20) and an interval beyond the
+    # run (one segment of 200 iterations, sized by the run, not by K)
+    grefs = [g.grad.numpy() for g in st]
+    for K in (30, 500):
+        run.forward(*c["st"], 200, K)
+        gK = run.backward(1.0)
+        for a, b, gr in zip(g1, gK, grefs):
+            assert rel_l2(b, a) < 1e-9, K
+            assert rel_l2(b, gr) < 1e-7, K
     run.close()
 
 

@@ -293,7 +293,9 @@ def test_fp32_close_to_fp64():
     assert rel_l2(d.field("Ex"), o.field("Ex")) < 1e-4
     assert_same_particles(o, d, 2, exact=False, tol=1e-5)
     e = d.energy()
-    assert rel_l2(e["field"], W) < 1e-4
+    err = rel_l2(e["field"], W)
+    print(f"FP32 field-energy history vs the FP64 oracle, 20 steps: rel L2 {err:.2e}")
+    assert err < 1e-5
 
 
 def test_umax_violation_reported():
@@ -304,6 +306,34 @@ def test_umax_violation_reported():
     with pytest.raises(ca.CpicError) as ei:
         d.sync()
     assert ei.value.code == -3
+
+
+def test_failed_resort_leaves_a_handle_that_refuses_work(monkeypatch):
+    """A re-sort that fails part way (here a particle-buffer allocation,
+    made to fail from the host) frees a buffer the push writes: the call
+    returns CPIC_ENOMEM and every later call on the handle but destroy
+    returns CPIC_ESTATE, without launching a kernel."""
+    o, d = c1(seed=4)
+    d.pre_step()
+    for it in range(3):
+        d.step(it)
+    d.sync()
+    # eight times the particles of species 0: the next commit must allocate
+    ids, x, y, ux, uy = two_stream(8 * 5120, (64.0, 64.0), 0.1, 0.01, 5)[0]
+    d.upload(0, ids, x, y, ux, uy)
+    monkeypatch.setenv("CPIC_FAIL_PARTS_ALLOC", "1")
+    with pytest.raises(ca.CpicError) as ei:
+        d.rebuild()
+    assert ei.value.code == -2
+    monkeypatch.delenv("CPIC_FAIL_PARTS_ALLOC")
+    calls = [lambda: d.step(3), lambda: d.run(3, 4), d.sync, lambda: d.field("rho"),
+             lambda: d.particles(1), d.rebuild, d.pre_step, lambda: d.stage("plasma_r", 3),
+             lambda: d.upload(1, ids, x, y, ux, uy), d.energy, d.solve_xy]
+    for k, call in enumerate(calls):
+        with pytest.raises(ca.CpicError) as ei:
+            call()
+        assert ei.value.code == -4, k
+    d.close()
 
 
 def test_in_plane_B_rejected():
