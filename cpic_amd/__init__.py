@@ -17,14 +17,18 @@ F_IDS, F_PARTICLE_E, F_PROFILE, F_NO_DIAG, F_NO_GRAPH = 0x1, 0x2, 0x4, 0x8, 0x10
 RHO, PHI, EX, EY = 0, 1, 2, 3
 FIELDS = {"rho": RHO, "phi": PHI, "Ex": EX, "Ey": EY}
 ERRORS = {0: "CPIC_OK", -1: "CPIC_EINVAL", -2: "CPIC_ENOMEM", -3: "CPIC_EUMAX",
-          -4: "CPIC_ESTATE", -5: "CPIC_EHIP", -6: "CPIC_ENODEV", -7: "CPIC_ENAN"}
+          -4: "CPIC_ESTATE", -5: "CPIC_EHIP", -6: "CPIC_ENODEV", -7: "CPIC_ENAN",
+          -8: "CPIC_ESIZE"}
+# Bytes of the static per-GPU workspace of cpic_solve_xy_noalloc.
+NOALLOC_WORKSPACE_BYTES = 34078720
 
 # Every symbol include/cpic_hip.h declares.
 EXPORTS = (
     "cpic_dev_strerror", "cpic_dev_create", "cpic_dev_destroy", "cpic_dev_upload_particles",
     "cpic_dev_download_particles", "cpic_dev_count", "cpic_dev_pre_step", "cpic_dev_field_E",
     "cpic_dev_plasma_E", "cpic_dev_plasma_r", "cpic_dev_field_rho", "cpic_dev_step",
-    "cpic_dev_run", "cpic_dev_sync", "cpic_dev_solve_xy", "cpic_solver_rho_size",
+    "cpic_dev_run", "cpic_dev_sync", "cpic_dev_solve_xy", "cpic_solve_xy_noalloc",
+    "cpic_solver_rho_size",
     "cpic_dev_download_field", "cpic_dev_upload_field", "cpic_dev_energy", "cpic_dev_migration",
     "cpic_dev_cell_counts", "cpic_dev_rebuild", "cpic_dev_stats", "cpic_dev_kernel_times",
     "cpic_dev_reset_kernel_times", "cpic_dev_set_profiling", "cpic_batch_create", "cpic_batch_size",
@@ -93,6 +97,7 @@ def lib():
         L.cpic_dev_run.argtypes = [vp, C.c_int64, C.c_int64]
         L.cpic_dev_sync.argtypes = [vp]
         L.cpic_dev_solve_xy.argtypes = [vp]
+        L.cpic_solve_xy_noalloc.argtypes = [C.c_int64, C.c_int64, vp, vp, vp]
         L.cpic_solver_rho_size.restype = C.c_int64
         L.cpic_solver_rho_size.argtypes = [C.c_int64, C.c_int64, i64, i64]
         L.cpic_dev_download_field.argtypes = [vp, C.c_int32, d]

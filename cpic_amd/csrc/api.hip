@@ -35,6 +35,7 @@ cpic_dev_strerror(int err)
 		case CPIC_EHIP: return "HIP runtime error";
 		case CPIC_ENODEV: return "no HIP device";
 		case CPIC_ENAN: return "non-finite particle state";
+		case CPIC_ESIZE: return "input larger than the fixed workspace of an allocation-free call";
 	}
 	return "unknown error";
 }

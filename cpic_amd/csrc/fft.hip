@@ -29,6 +29,7 @@
 #include "dev.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -501,7 +502,10 @@ void k_efield(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 
 // ---------------------------------------------------------------- host ---
 
-static void
+// exp(+2 pi i k/n), 0 <= k <= n/2, by octant reduction so that the
+// quarter points are exact (host tables; the allocation-free solve builds
+// its table on the device with the same reduction)
+static __host__ __device__ void
 twiddle(int64_t k, int64_t n, double *c, double *s)
 {
 	int64_t k8 = 8 * k;
@@ -526,6 +530,27 @@ twiddle(int64_t k, int64_t n, double *c, double *s)
 		a = 2.0 * M_PI * (double) (n / 2 - k) / (double) n;
 		*c = -cos(a); *s = sin(a);
 	}
+}
+
+// One row of 8192 points needs 64 KB + 16 B of LDS, a column 128 KB: the
+// passes' dynamic LDS limit is raised, leaving room for each kernel's
+// static LDS.
+static int
+fft_lds_attrs(void)
+{
+	const void *kern[] = {(const void *) k_rows_r2c, (const void *) k_rows_c2r,
+		(const void *) k_cols};
+	for(unsigned i = 0; i < sizeof(kern) / sizeof(kern[0]); i++)
+	{
+		hipFuncAttributes fa;
+		HIPCHK(hipFuncGetAttributes(&fa, kern[i]));
+		size_t lim = CPIC_LDS_LIMIT;
+		if(lim + fa.sharedSizeBytes > 160 * 1024)
+			lim = 160 * 1024 - fa.sharedSizeBytes;
+		HIPCHK(hipFuncSetAttribute(kern[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+				(int) lim));
+	}
+	return 0;
 }
 
 int
@@ -579,21 +604,7 @@ cpic_fft_init(cpic_dev *d)
 	free(tw);
 	if(e1 != hipSuccess || e2 != hipSuccess)
 		return CPIC_EHIP;
-	// one row of 8192 points needs 64 KB + 16 B of LDS, a column 128 KB;
-	// the dynamic limit leaves room for each kernel's static LDS
-	const void *kern[] = {(const void *) k_rows_r2c, (const void *) k_rows_c2r,
-		(const void *) k_cols};
-	for(unsigned i = 0; i < sizeof(kern) / sizeof(kern[0]); i++)
-	{
-		hipFuncAttributes fa;
-		HIPCHK(hipFuncGetAttributes(&fa, kern[i]));
-		size_t lim = CPIC_LDS_LIMIT;
-		if(lim + fa.sharedSizeBytes > 160 * 1024)
-			lim = 160 * 1024 - fa.sharedSizeBytes;
-		HIPCHK(hipFuncSetAttribute(kern[i], hipFuncAttributeMaxDynamicSharedMemorySize,
-				(int) lim));
-	}
-	return 0;
+	return fft_lds_attrs();
 }
 
 // FFT blocking of one plasma: R rows per row block (power of two dividing
@@ -782,6 +793,166 @@ cpic_efield(cpic_dev *d)
 				d->geo, d->pre + PRE_NODEBLK * (P + 1), P, d->phi, (float2 *) d->E, rz, dz,
 				d->ring, d->st);
 	cpic_kt_end(d, slot);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// ------------------------------------------------- allocation-free solve ---
+//
+// cpic_solve_xy_noalloc runs the three passes above on one grid for callers
+// that cannot allocate device memory.  Its spectrum, cosine and twiddle
+// tables and the passes' one-entry work lists live in `scratch`, a static
+// device array that each GPU holds from the library's load on that GPU to
+// the end of the process.  In host code the name `scratch` stands for a
+// host-side shadow, not a device address: the device address is looked up
+// once per GPU with hipGetSymbolAddress and kept in ws_addr[], and only that
+// address is handed to kernels.
+
+#define WS_GEO 0            // Geo of the grid
+#define WS_PRE 512          // row work list {0, rows}, column work list {0, cols}
+#define WS_STEP 768         // StepState (iter -1: the passes record no energy)
+#define WS_TABLES 1024      // cosines, twiddles, spectrum, each 256-aligned
+#define WS_MAX_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[CPIC_NOALLOC_WORKSPACE_BYTES];
+
+static_assert(sizeof(Geo) <= WS_PRE - WS_GEO, "Geo fits its workspace slot");
+static_assert(sizeof(StepState) <= WS_TABLES - WS_STEP, "StepState fits its slot");
+
+static pthread_mutex_t ws_lock = PTHREAD_MUTEX_INITIALIZER;
+static unsigned char *ws_addr[WS_MAX_DEVICES];
+
+// Writes the work lists and tables of one solve into the workspace: every
+// entry the passes read is rewritten, so what an earlier call left there
+// does not matter.  The tables are the host's expressions (cos(2 pi k/n) as
+// MFT_init computes them, src/solver.c:258-280; the octant-reduced
+// twiddles) evaluated with the device's cos and sin.
+__global__ __launch_bounds__(CPIC_BLOCK)
+void k_noalloc_setup(Geo g, int rows, int cols, int lgN, unsigned char *__restrict__ ws,
+		double *__restrict__ costab, double2 *__restrict__ tw)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int stride = gridDim.x * blockDim.x;
+	if(i == 0)
+	{
+		*(Geo *) (ws + WS_GEO) = g;
+		int *pre = (int *) (ws + WS_PRE);
+		pre[0] = 0;
+		pre[1] = rows;
+		pre[2] = 0;
+		pre[3] = cols;
+		StepState *st = (StepState *) (ws + WS_STEP);
+		st->iter = -1;
+	}
+	int hx = g.nx / 2 + 1, N = 1 << lgN;
+	double cx = 2.0 * M_PI / g.nx, cy = 2.0 * M_PI / g.ny;
+	for(int k = i; k < hx; k += stride)
+		costab[k] = cos(cx * k);
+	for(int k = i; k < g.ny; k += stride)
+		costab[hx + k] = cos(cy * k);
+	for(int k = i; k <= N / 2; k += stride)
+		twiddle(k, N, &tw[k].x, &tw[k].y);
+}
+
+// Device address of this GPU's workspace (the current device), looked up
+// on the first call on each GPU.
+static int
+ws_get(unsigned char **out)
+{
+	int dev = 0, ret = 0;
+	*out = NULL;
+	HIPCHK(hipGetDevice(&dev));
+	if(dev < 0 || dev >= WS_MAX_DEVICES)
+		return CPIC_EINVAL;
+	pthread_mutex_lock(&ws_lock);
+	if(!ws_addr[dev])
+	{
+		void *p = NULL;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !p)
+		{
+			fprintf(stderr, "cpic_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+					dev, hipGetErrorString(e));
+			ret = CPIC_EHIP;
+		}
+		else if(!(ret = fft_lds_attrs()))
+			ws_addr[dev] = (unsigned char *) p;
+	}
+	*out = ws_addr[dev];
+	pthread_mutex_unlock(&ws_lock);
+	return ret;
+}
+
+static size_t
+align256(size_t n)
+{
+	return (n + 255) & ~(size_t) 255;
+}
+
+extern "C" int
+cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, void *stream)
+{
+	int ret;
+	if(!is_pow2(nx) || !is_pow2(ny) || nx < 2 || ny < 2 || nx > 8192 || ny > 8192 ||
+			!rho || !phi)
+		return CPIC_EINVAL;
+	// the blocking of a batch of one (batch_create)
+	Geo g;
+	memset(&g, 0, sizeof(g));
+	g.nx = (int) nx;
+	g.ny = (int) ny;
+	g.lgnx = ilog2(nx);
+	g.lgny = ilog2(ny);
+	cpic_fft_blocking(&g);
+	if(!cpic_fft_blocking_ok(&g))
+		return CPIC_EINVAL;
+	cpic_fft_spread(&g, 1);
+
+	int64_t hx = nx / 2 + 1, N = nx > ny ? nx : ny;
+	size_t o_cos = WS_TABLES;
+	size_t o_tw = align256(o_cos + sizeof(double) * (size_t) (hx + ny));
+	size_t o_ghat = align256(o_tw + sizeof(double2) * (size_t) (N / 2 + 1));
+	size_t need = o_ghat + sizeof(double2) * (size_t) hx * (size_t) ny;
+	if(need > (size_t) CPIC_NOALLOC_WORKSPACE_BYTES)
+	{
+		fprintf(stderr, "cpic_solve_xy_noalloc: a %lld x %lld grid needs %zu bytes of "
+				"workspace, more than the fixed %zu\n", (long long) nx, (long long) ny,
+				need, (size_t) CPIC_NOALLOC_WORKSPACE_BYTES);
+		return CPIC_ESIZE;
+	}
+	unsigned char *ws;
+	if((ret = ws_get(&ws)))
+		return ret;
+
+	hipStream_t s = (hipStream_t) stream;
+	int lgN = ilog2(N);
+	int rows = (int) (ny / g.R), cols = (int) ((hx + g.C - 1) / g.C);
+	size_t lds_rows = sizeof(double2) * (size_t) g.R * (size_t) hx;
+	size_t lds_cols = sizeof(double2) * (size_t) g.C * (size_t) ny;
+	const Geo *geo = (const Geo *) (ws + WS_GEO);
+	const int *pre = (const int *) (ws + WS_PRE);
+	const StepState *st = (const StepState *) (ws + WS_STEP);
+	double *costab = (double *) (ws + o_cos);
+	double2 *tw = (double2 *) (ws + o_tw);
+	double2 *ghat = (double2 *) (ws + o_ghat);
+	int64_t nset = hx > ny ? hx : ny;
+	if(N / 2 + 1 > nset)
+		nset = N / 2 + 1;
+	nset = (nset + CPIC_BLOCK - 1) / CPIC_BLOCK;
+
+	hipLaunchKernelGGL(k_noalloc_setup, dim3((unsigned) nset), dim3(CPIC_BLOCK), 0, s, g,
+			rows, cols, lgN, ws, costab, tw);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_rows_r2c, dim3(rows), dim3(FFT_THREADS), lds_rows, s,
+			geo, pre, 1, rho, ghat, (const double2 *) tw, lgN);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_cols, dim3(cols), dim3(FFT_THREADS), lds_cols, s,
+			geo, pre + 2, 1, ghat, (const double *) costab, (const double2 *) tw, lgN,
+			(DiagSlot *) NULL, 1, st);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_rows_c2r, dim3(rows), dim3(FFT_THREADS), lds_rows, s,
+			geo, pre, 1, (const double2 *) ghat, phi, (const double2 *) tw, lgN,
+			(double *) NULL, (DiagSlot *) NULL, 1, st);
 	HIPCHK(hipGetLastError());
 	return 0;
 }

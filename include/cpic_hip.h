@@ -35,6 +35,8 @@ extern "C" {
 #define CPIC_EHIP    -5   /* HIP runtime error */
 #define CPIC_ENODEV  -6   /* no HIP device */
 #define CPIC_ENAN    -7   /* non-finite particle state detected */
+#define CPIC_ESIZE   -8   /* input larger than the fixed workspace of an
+                             allocation-free call */
 
 /* desc.flags */
 #define CPIC_F_IDS         0x1   /* carry the particle global index (ppack.i, src/def.h:93) */
@@ -157,6 +159,21 @@ int cpic_dev_sync(cpic_dev_t *dev);
 /* solve_xy with the MFT method (src/solver.c:599-621, MFT_solve :466-509):
  * phi from the current rho, without the E stencil. */
 int cpic_dev_solve_xy(cpic_dev_t *dev);
+
+/* Allocation-free variant of cpic_dev_solve_xy.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  phi = the MFT solve of rho for one nx x ny grid (dense,
+ * row-major, X fastest; powers of two), with rho and phi device pointers of
+ * the current device, queued on `stream` (a hipStream_t; NULL = the null
+ * stream); returns once queued.  It calls no allocator: the spectrum and the
+ * tables live in a static workspace of CPIC_NOALLOC_WORKSPACE_BYTES that
+ * each GPU holds until the process ends (enough for 2048 x 2048; grids that
+ * need more return CPIC_ESIZE).  All calls on one GPU share that workspace:
+ * calls that overlap on one GPU (other streams, other host threads) race. */
+#define CPIC_NOALLOC_WORKSPACE_BYTES 34078720   /* 32.5 MiB */
+int cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi,
+		void *stream);
 
 /* solver_rho_size (src/solver.c:382-433) for one rank: the FFTW-padded rho
  * shape 2*(nx/2+1) x ny, in doubles.  Returns the element count. */
