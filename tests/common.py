@@ -72,3 +72,31 @@ def growth_rate(log_energy, dt, window):
     t = np.arange(a, b) * dt
     slope = np.polyfit(t, np.asarray(log_energy)[a:b], 1)[0]
     return 0.5 * slope
+
+
+def c3_workload(grid, ppc=64):
+    """Electrons + mobile ions at 1:1836 on a grid x grid periodic domain with
+    unit cells: half the particles are electrons (q = -1, m = n_e per cell so
+    that w_p = 1) counter-streaming at +-0.1 with a 0.01 Gaussian spread, half
+    are ions (q = +1, m = 1836 m_e, spread 0.01/sqrt(1836)); B = 0, dt = 0.05."""
+    n_cells = grid * grid
+    n_e = n_cells * ppc // 2
+    m_e = n_e / n_cells
+    return dict(grid=grid, L=float(grid), n=(n_e, n_cells * ppc - n_e), q=(-1.0, 1.0),
+                m=(m_e, 1836.0 * m_e), v0=(0.1, 0.0), sigma=(0.01, 0.01 / np.sqrt(1836.0)),
+                dt=0.05)
+
+
+def c3_species(w, s, seed, ids=True):
+    """Species s of a c3_workload: (ids, x, y, ux, uy), positions uniform,
+    streams alternating +-v0 by index."""
+    rng = np.random.default_rng([seed, s])
+    n, L = w["n"][s], w["L"]
+    x = rng.uniform(0.0, L, n)
+    y = rng.uniform(0.0, L, n)
+    ux = rng.normal(0.0, w["sigma"][s], n)
+    if w["v0"][s]:
+        ux[0::2] += w["v0"][s]
+        ux[1::2] -= w["v0"][s]
+    uy = rng.normal(0.0, w["sigma"][s], n)
+    return (np.arange(n, dtype=np.int64) if ids else np.empty(0, np.int64)), x, y, ux, uy

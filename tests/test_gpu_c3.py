@@ -1,0 +1,111 @@
+"""The mixed-precision workload (electrons + mobile ions at 1:1836, FP32
+tile-relative particles, FP32 push, FP64 Poisson solve and diagnostics)
+against the FP64 oracle.
+
+- Scaled to 256^2 at 64 particles per cell (4,194,304 particles), with the
+  production shape of the full-size run (32x16 tiles, 4 particles per
+  thread): after 20 steps rho, E and the particle state are within 1e-5
+  relative L2 of the oracle's, and the field-energy history of 1000 steps
+  is within 1e-5 (max relative error per step) of the oracle's history.
+- The full size, 2048^2 with 268,435,456 particles, for 50 steps replayed
+  from captured graphs: invariants only (the oracle would need ~15 minutes
+  of host time): per-species counts from the device's cell histogram
+  unchanged, total charge zero to rounding, no non-finite state (the push
+  latches CPIC_ENAN), no particle outside its tile."""
+import time
+
+import numpy as np
+import pytest
+
+import cpic_amd as ca
+import pyoracle as po
+from common import c3_species, c3_workload, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def sorted_by_id(p):
+    order = np.argsort(p["id"], kind="stable")
+    return {k: v[order] for k, v in p.items()}
+
+
+def test_c3_scaled_fp32_vs_fp64_oracle():
+    w = c3_workload(256)
+    t0 = time.time()
+    grid, L = w["grid"], (w["L"], w["L"])
+    o = po.Oracle(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), nchunks=64)
+    d = ca.Device(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), precision=32,
+                  tile=(32, 16), ppt=4, flags=ca.F_IDS)
+    for s in range(2):
+        ids, x, y, ux, uy = c3_species(w, s, seed=2024)
+        o.add(s, ids, x, y, ux, uy)
+        d.upload(s, ids, x, y, ux, uy)
+    print(f"setup {time.time() - t0:.1f} s, {sum(w['n'])} particles", flush=True)
+    o.pre_step()
+    d.pre_step()
+    nsteps, W = 1000, []
+    for it in range(nsteps):
+        o.stage("field_E")
+        W.append(o.field_energy())
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        o.iter = it + 1
+        if it < 20:
+            d.step(it)
+        if it == 19:
+            d.sync()
+            errs = {f: rel_l2(d.field(f), o.field(f)) for f in ("rho", "phi", "Ex", "Ey")}
+            for s in range(2):
+                a, b = sorted_by_id(o.particles(s)), d.particles(s)
+                assert np.array_equal(a["id"], b["id"])
+                for k in ("x", "y", "ux", "uy"):
+                    errs[f"{s}.{k}"] = rel_l2(b[k], a[k])
+            print("after 20 steps, rel L2 vs the FP64 oracle:",
+                  {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+            assert max(errs.values()) <= 1e-5, errs
+            d.run(20, nsteps - 20)
+        if it % 200 == 199:
+            print(f"oracle step {it + 1}: {time.time() - t0:.1f} s", flush=True)
+    d.sync()
+    W = np.array(W)
+    Wg = d.energy(maxn=nsteps)["field"]
+    assert len(Wg) == nsteps
+    rel = np.abs(Wg - W) / np.abs(W)
+    print(f"field energy over {nsteps} steps vs the FP64 oracle: max rel {rel.max():.2e} "
+          f"(first 100 steps {rel[:100].max():.2e}); W(end)/W(0) {W[-1] / W[0]:.3f}; "
+          f"{time.time() - t0:.1f} s", flush=True)
+    assert rel.max() <= 1e-5
+    d.close()
+
+
+def test_c3_full_size_invariants():
+    w = c3_workload(2048)
+    t0 = time.time()
+    grid, L = w["grid"], (w["L"], w["L"])
+    d = ca.Device(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), precision=32,
+                  tile=(32, 16), ppt=4)
+    for s in range(2):
+        ids, x, y, ux, uy = c3_species(w, s, seed=2024, ids=False)
+        d.upload(s, ids, x, y, ux, uy)
+        del ids, x, y, ux, uy
+    print(f"setup {time.time() - t0:.1f} s, {sum(w['n'])} particles", flush=True)
+    d.pre_step()
+    d.run(0, 50)
+    d.sync()                       # raises on CPIC_ENAN / CPIC_EUMAX
+    print(f"50 steps {time.time() - t0:.1f} s", flush=True)
+    st = d.stats()
+    assert st["misplaced"] == 0 and st["nparticles"] == sum(w["n"]), st
+    for s in range(2):
+        assert int(d.cell_counts(s).sum()) == w["n"][s], s
+    rho = d.field("rho")
+    phi = d.field("phi")
+    assert np.all(np.isfinite(rho)) and np.all(np.isfinite(phi))
+    # equal numbers of opposite unit charges: the deposit sums to zero up to
+    # the rounding of the FP64 atomics (each node's sum is O(sqrt(ppc)))
+    assert abs(rho.sum()) <= 1e-9 * np.abs(rho).sum(), rho.sum()
+    e = d.energy(maxn=50)
+    assert len(e["field"]) == 50
+    assert np.all(np.isfinite(e["field"])) and np.all(e["field"] > 0)
+    assert np.all(np.isfinite(e["kinetic"]))
+    d.close()

@@ -84,6 +84,59 @@ def test_constant_speed_through_host_driver():
     h.sim_free(s)
 
 
+def test_cyclotron_through_host_driver():
+    """test/cyclotron.c (conf/cyclotron.conf: one electron, q = -1, m = 1, in
+    B = (0, 0, -1) on a 4x4 grid, dt = 0.05, 5,000 steps) through
+    sim_load/sim_step.  With E = 0 the Boris step rotates u by exactly
+    -2 atan(q B dt / 2m) per step and keeps |u|, and the positions lie on a
+    circle of radius dt |u| / (2 sin(theta/2)).  The particle's own field is
+    zero only in exact arithmetic: with e0 = 8.85e-12 its rounding residual
+    (E ~ 1e-7 at the particle, the same in the oracle) bounds the checks at
+    1e-7 relative.  The centre is taken from the orbit itself (the
+    reference's test places it on the wrong side, SURVEY's caveat aside).
+    The trajectory also matches the oracle's step by step."""
+    import pyoracle as po
+    h = host()
+    with tempfile.TemporaryDirectory() as d:
+        s = h.sim_load(tracked_copy(os.path.join(CONF, "cyclotron.conf"), d).encode(), 1)
+    assert s
+    assert h.sim_get_cycles(s) == 5000
+    traj = []
+    while h.sim_get_iter(s) < h.sim_get_cycles(s):
+        assert h.sim_step(s) == 0
+        traj.append(particle(h, s, 0, 0)[:4])
+    h.sim_free(s)
+    traj = np.array(traj)
+    o = po.Oracle(4, 4, (32.0, 32.0), 0.05, [-1.0], [1.0], e0=8.85e-12, B=(0.0, 0.0, -1.0),
+                  nchunks=4)
+    o.add(0, np.array([0]), np.array([2.0]), np.array([16.0]), np.array([0.0]), np.array([10.0]))
+    o.pre_step()
+    ref = []
+    for _ in range(5000):
+        o.step()
+        p = o.particles(0)
+        ref.append([p["x"][0], p["y"][0], p["ux"][0], p["uy"][0]])
+    ref = np.array(ref)
+    assert np.max(np.abs(traj - ref)) < 1e-5, np.max(np.abs(traj - ref), axis=0)
+    speed = np.hypot(traj[:, 2], traj[:, 3])
+    assert np.max(np.abs(speed - 10.0)) <= 1e-7 * 10.0
+    theta = -2.0 * np.arctan((-1.0) * (-1.0) * 0.05 / (2.0 * 1.0))
+    dang = np.diff(np.unwrap(np.arctan2(traj[:, 3], traj[:, 2])))
+    assert np.max(np.abs(dang - theta)) <= 1e-7, (dang.min(), dang.max(), theta)
+    # circumcentre of three positions after the first move
+    (ax, ay), (bx, by), (cx, cy) = traj[1, :2], traj[200, :2], traj[400, :2]
+    dd = 2 * (ax * (by - cy) + bx * (cy - ay) + cx * (ay - by))
+    ux_ = ((ax * ax + ay * ay) * (by - cy) + (bx * bx + by * by) * (cy - ay) +
+           (cx * cx + cy * cy) * (ay - by)) / dd
+    uy_ = ((ax * ax + ay * ay) * (cx - bx) + (bx * bx + by * by) * (ax - cx) +
+           (cx * cx + cy * cy) * (bx - ax)) / dd
+    R = 0.05 * 10.0 / (2 * np.sin(abs(theta) / 2))
+    dist = np.hypot(traj[1:, 0] - ux_, traj[1:, 1] - uy_)
+    assert np.max(np.abs(dist - R)) <= 1e-6 * R, (np.max(np.abs(dist - R)), R)
+    # clockwise about a centre to the right of the start (x0 + R, y0)
+    assert abs(ux_ - (2.0 + R)) < 1e-3 and abs(uy_ - 16.0) < 0.5
+
+
 def test_cpic_cli_stats_line():
     """cpic prints the reference's stats line when sampling (src/sim.c:465)."""
     with tempfile.TemporaryDirectory() as d:
