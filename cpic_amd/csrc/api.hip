@@ -297,7 +297,6 @@ geo_fill(const cpic_dev_desc_t *desc, Geo *g)
 		g->q[s] = desc->q[s];
 		g->m[s] = desc->m[s];
 	}
-	cpic_fft_blocking(g);
 }
 
 static int
@@ -357,10 +356,19 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 	int *pt = d->hpre + PRE_TILE * (P + 1), *pr = d->hpre + PRE_ROW * (P + 1);
 	int *pc = d->hpre + PRE_COL * (P + 1), *pn = d->hpre + PRE_NODEBLK * (P + 1);
 	int *pb = d->hpre + PRE_BUCKET * (P + 1);
+	int64_t mmax = 1, nymax = 1;
 	for(p = 0; p < P; p++)
 	{
 		geo_fill(&descs[p], &d->hgeo[p]);
-		if(!cpic_fft_blocking_ok(&d->hgeo[p]))
+		if(d->hgeo[p].nx / 2 > mmax) mmax = d->hgeo[p].nx / 2;
+		if(d->hgeo[p].ny > nymax) nymax = d->hgeo[p].ny;
+	}
+	// one block size per FFT pass for the whole batch
+	cpic_fft_threads(mmax, nymax, &d->fft_nt_row, &d->fft_nt_col);
+	for(p = 0; p < P; p++)
+	{
+		cpic_fft_blocking(&d->hgeo[p], d->fft_nt_row, d->fft_nt_col);
+		if(!cpic_fft_blocking_ok(&d->hgeo[p], d->fft_nt_row, d->fft_nt_col))
 		{
 			dev_free(d);
 			return CPIC_EINVAL;
@@ -388,7 +396,7 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 		tiles += g->ntiles;
 		buckets += (int64_t) g->ntiles * g->ns;
 		rows += g->ny / g->R;
-		cols += (hx + g->C - 1) / g->C;
+		cols += (g->nx / 2 + g->C - 1) / g->C;   // kx = 0 and nx/2 share a column
 		nblk += (n + EF_NODES - 1) / EF_NODES;
 		size_t lr = sizeof(double2) * (size_t) g->R * (size_t) hx;
 		size_t lc = sizeof(double2) * (size_t) g->C * (size_t) g->ny;

@@ -145,6 +145,7 @@ struct cpic_dev {
 	int ntiles;              // sum of tiles
 	int nb;                  // sum of buckets
 	int rowblocks, colblocks, nodeblocks;
+	int fft_nt_row, fft_nt_col;   // threads per block of the FFT row and column passes
 	size_t lds_push, lds_rows, lds_cols;
 
 	// fields
@@ -209,9 +210,10 @@ struct cpic_dev {
 
 // helpers implemented in the .hip files
 int cpic_fft_init(cpic_dev *d);
-void cpic_fft_blocking(Geo *g);
+void cpic_fft_threads(int64_t mmax, int64_t nymax, int *nt_row, int *nt_col);
+void cpic_fft_blocking(Geo *g, int nt_row, int nt_col);
 void cpic_fft_spread(Geo *g, int P);
-int cpic_fft_blocking_ok(const Geo *g);
+int cpic_fft_blocking_ok(const Geo *g, int nt_row, int nt_col);
 size_t cpic_push_lds(const cpic_dev *d);
 int cpic_set_iter(cpic_dev *d, int64_t iter);
 int cpic_solve(cpic_dev *d, int want_energy);
@@ -270,6 +272,24 @@ void cpic_kt_end(cpic_dev *d, int slot);
 #define FFT_THREADS 1024
 #define FFT_MIN_BLOCKS 256       // blocks per FFT pass worth spreading to (one per CU)
 #define FFT_MAX_VALUES 8192
+// Smaller blocks for the passes whose rows / columns fit them (more blocks
+// resident per CU, cheaper barriers): threads per row and column block,
+// values per thread in a row block, and their LDS budgets.
+#ifndef FFT_ROW_NT
+#define FFT_ROW_NT 1024
+#endif
+#ifndef FFT_ROW_VPT
+#define FFT_ROW_VPT 8
+#endif
+#ifndef FFT_ROW_LDS_SMALL
+#define FFT_ROW_LDS_SMALL (76 * 1024)
+#endif
+#ifndef FFT_COL_NT
+#define FFT_COL_NT 256
+#endif
+#ifndef FFT_COL_LDS_SMALL
+#define FFT_COL_LDS_SMALL (32 * 1024)
+#endif
 
 static inline int ilog2(int64_t n)
 {

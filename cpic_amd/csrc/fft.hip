@@ -16,16 +16,22 @@
 //                   nx*ny is a power of two so the scaling is exact), the field
 //                   energy partial sum, inverse FFT along ky.
 //   3. k_rows_c2r   R rows per workgroup: Hermitian pre-processing + inverse
-//                   half-length FFT; writes phi rows.
-// then k_efield writes (Ex,Ey) pairs in the particle kernels' precision and
-// zeroes the rho buffer the next push deposits into.
+//                   half-length FFT; writes phi rows, and zeroes the same rows
+//                   of the rho buffer the next push deposits into and the next
+//                   step's diagnostic slot.
+// E = -grad(phi) is computed by the push for its tiles from phi (in LDS);
+// k_efield writes the E array only when something reads it.
 //
-// FFT stages are radix-2 in LDS with twiddles from one table of
-// exp(-2 pi i k/N), N = the largest grid edge of the batch, built on the host
-// with an octant reduction so the quarter points are exact.  Each pass runs
-// once for all plasmas of a batch: block b takes the b-th item of the
-// pass's work list (row block, column block or node block) and finds its
-// plasma by binary search.
+// The transforms are Stockham autosort FFTs in LDS (natural order in and
+// out): radix-8 stages after one radix-2 or radix-4 stage for lg n mod 3,
+// each thread holding up to FFT_MAXV complex values in registers per stage.
+// Twiddles: each workgroup copies two small factor tables into LDS from one
+// half table of exp(+2 pi i k/N), N = the largest grid edge of the batch,
+// built with an octant reduction so the quarter points are exact; a
+// twiddle is then the product of two LDS entries.  The block size of each
+// pass is chosen per batch (cpic_fft_threads).  Each pass runs once for all
+// plasmas of a batch: block b takes the b-th item of the pass's work list
+// (row block or column block) and finds its plasma by binary search.
 #include "dev.h"
 
 #include <math.h>
@@ -147,9 +153,10 @@ static __device__ __forceinline__ void dft_small(double2 *v, double sign)
 	}
 }
 
-// complex values a thread holds during one FFT stage
+// complex values a thread holds during one FFT stage: a block of NT threads
+// transforms at most FFT_MAXV * NT values
 #define FFT_MAXV 8
-// two blocks of FFT_THREADS per CU: at most 64 VGPRs
+// at most 64 VGPRs per thread (eight waves per SIMD)
 #define FFT_MIN_WAVES 8
 static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
 
@@ -158,8 +165,8 @@ static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
 // transform reads x[i + t n/R], applies w^(t k) with w = exp(sign 2 pi i/(R p)),
 // k = i mod p, and writes the R-point DFT to y[(i - k) R + k + t p].  All
 // reads precede a barrier and all writes follow it, so one LDS buffer holds
-// both.  Needs (n/R)*nbat <= (FFT_MAXV/R) * FFT_THREADS.
-template <int R>
+// both.  Needs (n/R)*nbat <= (FFT_MAXV/R) * NT.
+template <int R, int NT>
 static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
 		int nbat, int lgp, const TwRef &tw, int lgN, double sign)
 {
@@ -173,7 +180,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 #pragma unroll
 	for(int m = 0; m < MAXB; m++)
 	{
-		int t_ = threadIdx.x + m * FFT_THREADS;
+		int t_ = threadIdx.x + m * NT;
 		jo[m] = -1;
 		if(t_ < total)
 		{
@@ -207,6 +214,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 // order in and out (Stockham autosort: radix-8 stages after one radix-2 or
 // radix-4 stage for lg n mod 3), w = exp(sign 2 pi i/len).  Starts and ends
 // at a barrier.
+template <int NT>
 static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
 		const TwRef &tw, int lgN, double sign)
 {
@@ -214,20 +222,21 @@ static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
 	int lgp = 0;
 	if(lgn % 3 == 1)
 	{
-		stockham_stage<2>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		stockham_stage<2, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 		lgp = 1;
 	}
 	else if(lgn % 3 == 2)
 	{
-		stockham_stage<4>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		stockham_stage<4, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 		lgp = 2;
 	}
 	for(; lgp < lgn; lgp += 3)
-		stockham_stage<8>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		stockham_stage<8, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
 }
 
-// Pass 1: R rows per block; row y real of length nx = 2m.
-__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
+// Pass 1: R rows per block of NT threads; row y real of length nx = 2m.
+template <int NT>
+__global__ __launch_bounds__(NT, FFT_MIN_WAVES)
 void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const double *__restrict__ rho_all, double2 *__restrict__ ghat_all,
 		const double2 *__restrict__ tw, int lgN)
@@ -252,7 +261,7 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_block(sh, stride, lgm, R, twr, lgN, -1.0);
+		fft_block<NT>(sh, stride, lgm, R, twr, lgN, -1.0);
 
 	// X[k] = E + W^k O, X[m-k] = conj(E - W^k O), W = exp(-2 pi i/nx)
 	int npair = (m >> 1) + 1;
@@ -294,15 +303,22 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 }
 
-// Pass 2: C columns (kx) per block, each ny long and contiguous.
-__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
+// Pass 2: C columns (kx) per block of NT threads, each ny long and contiguous.
+// Columns kx = 0 and kx = nx/2 of a real field are real sequences along y:
+// the first block of a plasma transforms them together as one complex
+// column (kx = 0 in the real part, kx = nx/2 in the imaginary part) and
+// separates their spectra by symmetry, so a plasma has nx/2 columns to
+// transform, not nx/2 + 1 (1024 columns at 2048^2: the blocks fill the chip
+// in one round instead of leaving one block for a second).
+template <int NT>
+__global__ __launch_bounds__(NT, FFT_MIN_WAVES)
 void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		double2 *__restrict__ ghat_all, const double *__restrict__ costab_all,
 		const double2 *__restrict__ tw, int lgN, DiagSlot *__restrict__ diag,
 		int ring, const StepState *__restrict__ st)
 {
 	extern __shared__ double2 sh[];
-	__shared__ double red[FFT_THREADS / 64];
+	__shared__ double red[NT / 64];
 	int p = find_plasma(pre, P, blockIdx.x);
 	const Geo g = geo[p];
 	int nx = g.nx, ny = g.ny, C = g.C;
@@ -310,7 +326,8 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	int m = nx >> 1;
 	int lgny = g.lgny;
 	int k0 = (blockIdx.x - pre[p]) * C;
-	int ncol = min(C, m + 1 - k0);
+	int ncol = min(C, m - k0);
+	const bool packed = k0 == 0;   // column 0 carries kx = 0 and kx = m
 	const double *cosx = costab_all + g.cos0, *cosy = cosx + (m + 1);
 	double invN = 1.0 / ((double) nx * (double) ny);
 	long long iter = st->iter;
@@ -320,22 +337,59 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-		sh[i] = ghat[(int64_t) k0 * ny + i];
+	{
+		if(packed && i < ny)
+			sh[i] = make_double2(ghat[i].x, ghat[(int64_t) m * ny + i].x);
+		else
+			sh[i] = ghat[(int64_t) k0 * ny + i];
+	}
 	__syncthreads();
 	if(lgny > 0)
-		fft_block(sh, ny, lgny, ncol, twr, lgN, -1.0);
+		fft_block<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0);
 
+	// G exactly as MFT_init computes it (src/solver.c:269-280)
+	auto green = [&](int kx, int ky) {
+		return (kx == 0 && ky == 0) ? 0.0 : 1.0 / (2.0 * (cosx[kx] + cosy[ky]) - 4.0);
+	};
 	double e = 0.0;
-	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
+	// the packed column's ky pairs (ky, ny - ky) first (one thread per pair),
+	// then the other columns' values
+	const int npair = packed ? (ny >> 1) + 1 : 0;
+	for(int j = threadIdx.x; j < npair + (ncol - (packed ? 1 : 0)) * ny; j += blockDim.x)
 	{
+		if(j < npair)
+		{
+			// Z = A + iB with A, B the spectra of the real columns kx = 0
+			// and kx = m: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] -
+			// conj Z[-k])/(2i), A[-k] = conj A[k], B[-k] = conj B[k]
+			int k = j, kn = (ny - j) & (ny - 1);
+			double2 zk = sh[k], zn = sh[kn];
+			double2 A = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+			double2 B = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+			double a2 = A.x * A.x + A.y * A.y, b2 = B.x * B.x + B.y * B.y;
+			double ga = green(0, k), gb = green(m, k);
+			e += ga * a2 + gb * b2;
+			ga *= invN;
+			gb *= invN;
+			// Z'[k] = ga A + i gb B
+			sh[k] = make_double2(ga * A.x - gb * B.y, ga * A.y + gb * B.x);
+			if(kn != k)
+			{
+				double gan = green(0, kn), gbn = green(m, kn);
+				e += gan * a2 + gbn * b2;
+				gan *= invN;
+				gbn *= invN;
+				// Z'[-k] = gan conj(A) + i gbn conj(B)
+				sh[kn] = make_double2(gan * A.x + gbn * B.y, -gan * A.y + gbn * B.x);
+			}
+			continue;
+		}
+		int i = j - npair + (packed ? ny : 0);
 		int c = i >> lgny, ky = i & (ny - 1);
 		int kx = k0 + c;
-		// G exactly as MFT_init computes it
-		double G = (kx == 0 && ky == 0) ? 0.0 :
-			1.0 / (2.0 * (cosx[kx] + cosy[ky]) - 4.0);
+		double G = green(kx, ky);
 		double2 v = sh[i];
-		double wk = (kx == 0 || kx == m) ? 1.0 : 2.0;
-		e += wk * G * (v.x * v.x + v.y * v.y);
+		e += 2.0 * G * (v.x * v.x + v.y * v.y);
 		double g = G * invN;
 		sh[i] = make_double2(v.x * g, v.y * g);
 	}
@@ -350,22 +404,32 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		if(threadIdx.x == 0)
 		{
 			double t = 0.0;
-			for(int w = 0; w < FFT_THREADS / 64; w++)
+			for(int w = 0; w < NT / 64; w++)
 				t += red[w];
 			unsafeAtomicAdd(energy, t * escale);
 		}
 	}
 	__syncthreads();
 	if(lgny > 0)
-		fft_block(sh, ny, lgny, ncol, twr, lgN, +1.0);
+		fft_block<NT>(sh, ny, lgny, ncol, twr, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-		ghat[(int64_t) k0 * ny + i] = sh[i];
+	{
+		double2 v = sh[i];
+		if(packed && i < ny)
+		{
+			ghat[i] = make_double2(v.x, 0.0);
+			ghat[(int64_t) m * ny + i] = make_double2(v.y, 0.0);
+		}
+		else
+			ghat[(int64_t) k0 * ny + i] = v;
+	}
 }
 
 // Pass 3: R rows per block, c2r along X (imaginary parts of X[0] and
 // X[nx/2] ignored, as for a Hermitian input).
-__global__ __launch_bounds__(FFT_THREADS, FFT_MIN_WAVES)
+template <int NT>
+__global__ __launch_bounds__(NT, FFT_MIN_WAVES)
 void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		const double2 *__restrict__ ghat_all, double *__restrict__ phi_all,
 		const double2 *__restrict__ tw, int lgN, double *__restrict__ rho_zero_all,
@@ -428,7 +492,7 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	}
 	__syncthreads();
 	if(lgm > 0)
-		fft_block(sh, stride, lgm, R, twr, lgN, +1.0);
+		fft_block<NT>(sh, stride, lgm, R, twr, lgN, +1.0);
 
 	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
 	{
@@ -538,8 +602,11 @@ twiddle(int64_t k, int64_t n, double *c, double *s)
 static int
 fft_lds_attrs(void)
 {
-	const void *kern[] = {(const void *) k_rows_r2c, (const void *) k_rows_c2r,
-		(const void *) k_cols};
+	const void *kern[] = {
+		(const void *) k_rows_r2c<256>, (const void *) k_rows_c2r<256>, (const void *) k_cols<256>,
+		(const void *) k_rows_r2c<512>, (const void *) k_rows_c2r<512>, (const void *) k_cols<512>,
+		(const void *) k_rows_r2c<1024>, (const void *) k_rows_c2r<1024>,
+		(const void *) k_cols<1024>};
 	for(unsigned i = 0; i < sizeof(kern) / sizeof(kern[0]); i++)
 	{
 		hipFuncAttributes fa;
@@ -607,17 +674,32 @@ cpic_fft_init(cpic_dev *d)
 	return fft_lds_attrs();
 }
 
-// FFT blocking of one plasma: R rows per row block (power of two dividing
-// ny, within the LDS budget, at most 8), C columns per column block.
+// Threads per block of the row and column passes for a batch whose largest
+// row holds mmax = nx/2 complex values and whose longest column ny = nymax:
+// FFT_ROW_NT / FFT_COL_NT threads when one row / column fits such a block
+// (FFT_MAXV values per thread), else FFT_THREADS.
 void
-cpic_fft_blocking(Geo *g)
+cpic_fft_threads(int64_t mmax, int64_t nymax, int *nt_row, int *nt_col)
 {
-	// rows: up to FFT_ROW_LDS_BYTES of LDS and FFT_THREADS*4 values (one
-	// radix-4 butterfly per thread), so two blocks share a CU
+	*nt_row = mmax <= (int64_t) FFT_MAXV * FFT_ROW_NT ? FFT_ROW_NT : FFT_THREADS;
+	*nt_col = nymax <= (int64_t) FFT_MAXV * FFT_COL_NT ? FFT_COL_NT : FFT_THREADS;
+}
+
+// FFT blocking of one plasma for row blocks of nt_row threads and column
+// blocks of nt_col threads: R rows per row block (a power of two dividing
+// ny), C columns per column block, each within the pass's values per
+// thread and LDS budget (FFT_THREADS blocks: 4 values per thread, 76 KB and
+// 64 KB, so that two share a CU).
+void
+cpic_fft_blocking(Geo *g, int nt_row, int nt_col)
+{
+	int64_t m = g->nx / 2 > 0 ? g->nx / 2 : 1;
 	int64_t per = (int64_t) (g->nx / 2 + 1) * (int64_t) sizeof(double2);
-	int64_t R = FFT_ROW_LDS_BYTES / per, r = 1;
-	if(R * (g->nx / 2) > FFT_THREADS * 4)
-		R = FFT_THREADS * 4 / (g->nx / 2 > 0 ? g->nx / 2 : 1);
+	int64_t vr = nt_row == FFT_THREADS ? 4 * FFT_THREADS : (int64_t) FFT_ROW_VPT * nt_row;
+	int64_t lr = nt_row == FFT_THREADS ? FFT_ROW_LDS_BYTES : FFT_ROW_LDS_SMALL;
+	int64_t R = lr / per, r = 1;
+	if(R * m > vr)
+		R = vr / m;
 	if(R < 1)
 		R = 1;
 	if(R > g->ny)
@@ -625,11 +707,15 @@ cpic_fft_blocking(Geo *g)
 	while(r * 2 <= R)
 		r *= 2;
 	g->R = (int) r;
-	int64_t C = FFT_LDS_BYTES / ((int64_t) g->ny * (int64_t) sizeof(double2));
+	int64_t vc = nt_col == FFT_THREADS ? 4 * FFT_THREADS : (int64_t) FFT_MAXV * nt_col;
+	int64_t lc = nt_col == FFT_THREADS ? FFT_LDS_BYTES : FFT_COL_LDS_SMALL;
+	int64_t C = lc / ((int64_t) g->ny * (int64_t) sizeof(double2));
+	if(C * g->ny > vc)
+		C = vc / g->ny;
 	if(C < 1)
 		C = 1;
-	if(C > g->nx / 2 + 1)
-		C = g->nx / 2 + 1;
+	if(C > g->nx / 2)
+		C = g->nx / 2;
 	if(C > 16)
 		C = 16;
 	g->C = (int) C;
@@ -663,7 +749,7 @@ cpic_fft_spread(Geo *g, int P)
 		int best = -1;
 		for(int p = 0; p < P; p++)
 		{
-			int64_t hx = g[p].nx / 2 + 1;
+			int64_t hx = g[p].nx / 2;     // columns to transform (kx = 0 and nx/2 packed)
 			cols += (hx + g[p].C - 1) / g[p].C;
 			if(g[p].C > 1 && (best < 0 || g[p].C > g[best].C))
 				best = p;
@@ -674,19 +760,30 @@ cpic_fft_spread(Geo *g, int P)
 	}
 }
 
-// Whether the blocking keeps every FFT pass within FFT_MAX_VALUES values and
-// the LDS limit per block.
+// Whether the blocking keeps every FFT pass within its blocks' values
+// (FFT_MAXV per thread) and the LDS limit per block.
 int
-cpic_fft_blocking_ok(const Geo *g)
+cpic_fft_blocking_ok(const Geo *g, int nt_row, int nt_col)
 {
 	// dynamic LDS, leaving 16 KB for the kernels' static LDS (twiddle
 	// factors, reduction partials)
 	const int64_t lim = 144 * 1024;
 	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= lim &&
 		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= lim &&
-		(int64_t) g->R * (g->nx / 2) <= FFT_MAX_VALUES &&
-		(int64_t) g->C * g->ny <= FFT_MAX_VALUES;
+		(int64_t) g->R * (g->nx / 2) <= (int64_t) FFT_MAXV * nt_row &&
+		(int64_t) g->C * g->ny <= (int64_t) FFT_MAXV * nt_col;
 }
+
+// Launch of an FFT pass with blocks of nt threads (the instantiations for
+// the thread counts cpic_fft_threads chooses: 256, 512 or 1024).
+#define FFT_LAUNCH(kern, nt, grid, lds, stream, ...) do { \
+		if((nt) == 256) \
+			hipLaunchKernelGGL(kern<256>, dim3(grid), dim3(256), lds, stream, __VA_ARGS__); \
+		else if((nt) == 512) \
+			hipLaunchKernelGGL(kern<512>, dim3(grid), dim3(512), lds, stream, __VA_ARGS__); \
+		else \
+			hipLaunchKernelGGL(kern<1024>, dim3(grid), dim3(1024), lds, stream, __VA_ARGS__); \
+	} while(0)
 
 // phi = S rho for every plasma of the batch (rho and phi laid out like the
 // state's own buffers); with diag, the field energy of each plasma goes to
@@ -700,20 +797,20 @@ solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag, double *rho_z
 	const int *pre = d->pre;
 
 	slot = cpic_kt_begin(d, "fft_rows_r2c");
-	hipLaunchKernelGGL(k_rows_r2c, dim3(d->rowblocks), dim3(FFT_THREADS), d->lds_rows, d->stream,
+	FFT_LAUNCH(k_rows_r2c, d->fft_nt_row, d->rowblocks, d->lds_rows, d->stream,
 			d->geo, pre + PRE_ROW * (P + 1), P, rho, d->ghat, d->tw, d->lgN);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
 	slot = cpic_kt_begin(d, "fft_cols_green");
-	hipLaunchKernelGGL(k_cols, dim3(d->colblocks), dim3(FFT_THREADS), d->lds_cols, d->stream,
+	FFT_LAUNCH(k_cols, d->fft_nt_col, d->colblocks, d->lds_cols, d->stream,
 			d->geo, pre + PRE_COL * (P + 1), P, d->ghat, d->costab, d->tw, d->lgN, diag,
 			d->ring, d->st);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
 	slot = cpic_kt_begin(d, "fft_rows_c2r");
-	hipLaunchKernelGGL(k_rows_c2r, dim3(d->rowblocks), dim3(FFT_THREADS), d->lds_rows, d->stream,
+	FFT_LAUNCH(k_rows_c2r, d->fft_nt_row, d->rowblocks, d->lds_rows, d->stream,
 			d->geo, pre + PRE_ROW * (P + 1), P, d->ghat, phi, d->tw, d->lgN, rho_zero,
 			diag_zero, d->ring, d->st);
 	cpic_kt_end(d, slot);
@@ -898,13 +995,15 @@ cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, vo
 		return CPIC_EINVAL;
 	// the blocking of a batch of one (batch_create)
 	Geo g;
+	int nt_row, nt_col;
 	memset(&g, 0, sizeof(g));
 	g.nx = (int) nx;
 	g.ny = (int) ny;
 	g.lgnx = ilog2(nx);
 	g.lgny = ilog2(ny);
-	cpic_fft_blocking(&g);
-	if(!cpic_fft_blocking_ok(&g))
+	cpic_fft_threads(nx / 2, ny, &nt_row, &nt_col);
+	cpic_fft_blocking(&g, nt_row, nt_col);
+	if(!cpic_fft_blocking_ok(&g, nt_row, nt_col))
 		return CPIC_EINVAL;
 	cpic_fft_spread(&g, 1);
 
@@ -926,7 +1025,7 @@ cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, vo
 
 	hipStream_t s = (hipStream_t) stream;
 	int lgN = ilog2(N);
-	int rows = (int) (ny / g.R), cols = (int) ((hx + g.C - 1) / g.C);
+	int rows = (int) (ny / g.R), cols = (int) ((nx / 2 + g.C - 1) / g.C);
 	size_t lds_rows = sizeof(double2) * (size_t) g.R * (size_t) hx;
 	size_t lds_cols = sizeof(double2) * (size_t) g.C * (size_t) ny;
 	const Geo *geo = (const Geo *) (ws + WS_GEO);
@@ -943,14 +1042,14 @@ cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, vo
 	hipLaunchKernelGGL(k_noalloc_setup, dim3((unsigned) nset), dim3(CPIC_BLOCK), 0, s, g,
 			rows, cols, lgN, ws, costab, tw);
 	HIPCHK(hipGetLastError());
-	hipLaunchKernelGGL(k_rows_r2c, dim3(rows), dim3(FFT_THREADS), lds_rows, s,
+	FFT_LAUNCH(k_rows_r2c, nt_row, rows, lds_rows, s,
 			geo, pre, 1, rho, ghat, (const double2 *) tw, lgN);
 	HIPCHK(hipGetLastError());
-	hipLaunchKernelGGL(k_cols, dim3(cols), dim3(FFT_THREADS), lds_cols, s,
+	FFT_LAUNCH(k_cols, nt_col, cols, lds_cols, s,
 			geo, pre + 2, 1, ghat, (const double *) costab, (const double2 *) tw, lgN,
 			(DiagSlot *) NULL, 1, st);
 	HIPCHK(hipGetLastError());
-	hipLaunchKernelGGL(k_rows_c2r, dim3(rows), dim3(FFT_THREADS), lds_rows, s,
+	FFT_LAUNCH(k_rows_c2r, nt_row, rows, lds_rows, s,
 			geo, pre, 1, (const double2 *) ghat, phi, (const double2 *) tw, lgN,
 			(double *) NULL, (DiagSlot *) NULL, 1, st);
 	HIPCHK(hipGetLastError());
