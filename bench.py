@@ -15,6 +15,12 @@ Prints one JSON line.  `value` = particle-steps/s of the FP32 step, timed
 over K steps of the library's production path (`cpic_dev_run`: steps
 replayed two at a time from a captured HIP graph); the per-kernel times
 (roofline) come from K further steps launched one by one between HIP events.
+The same line carries the FP64 step, the FP32 accuracy against the FP64 run
+(1000 steps, full size) and against the FP64 oracle (the same workload scaled
+to 256^2, 1000 steps; the oracle runs in the CPU-baseline leg), the step time
+late in the run (steps 900-1000), the HBM streaming ceiling of a plain
+read+write kernel, and the batched-ensemble and differentiable-run figures
+below as `ensemble` and `adjoint` objects.
 
 `--workload adjoint` times the differentiable FP64 run (256^2, 4,194,304
 particles, 200 steps, checkpoints every 20: forward and reverse sweep).
@@ -133,15 +139,53 @@ def oracle_sample(po, w, frac, steps, seed=7):
     return tf / steps, tp / steps, nsample
 
 
-def cpu_baseline(w, frac=8, steps=10):
+def oracle_history(po, w, seed, steps):
+    """The oracle's field-energy history over `steps` steps of workload w from
+    the seeded input make_device uploads (same generator, same seed)."""
+    grid = w["grid"]
+    o = po.Oracle(grid, grid, (w["L"], w["L"]), w["dt"], list(w["q"]), list(w["m"]), nchunks=min(64, grid))
+    for s in range(2):
+        rng = np.random.default_rng([seed, s])
+        x, y, ux, uy = gen_species(rng, w["n"][s], w["L"], w["v0"][s], w["sigma"][s])
+        o.add(s, np.arange(len(x), dtype=np.int64), x, y, ux, uy)
+        del x, y, ux, uy
+    o.pre_step()
+    hist = []
+    for it in range(steps):
+        o.stage("field_E")
+        hist.append(o.field_energy())
+        o.stage("plasma_E")
+        o.stage("plasma_r")
+        o.stage("field_rho")
+        o.iter = it + 1
+    del o
+    return np.array(hist)
+
+
+def cpu_baseline(w, frac=8, steps=10, accuracy=None):
     """The FP64 oracle (C restatement of the reference loop, OpenMP over 64
     chunks) on the host cores: the full grid with 1/frac of the particles;
     the full workload's step time is extrapolated as t_fields +
-    frac * t_particles.  Also one core on a 1/256 sample."""
+    frac * t_particles.  Also one core on a 1/256 sample.  With `accuracy`
+    (from fp32_oracle_leg_gpu), also the oracle's run of that scaled
+    workload, whose field-energy history is the FP32 accuracy reference
+    (stored in accuracy["oracle"]) and whose rate is reported beside."""
     import ctypes
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    acc_run = None
+    if accuracy is not None:
+        wa = accuracy["w"]
+        t0 = time.perf_counter()
+        accuracy["oracle"] = oracle_history(po, wa, accuracy["seed"], accuracy["steps"])
+        el = time.perf_counter() - t0
+        acc_run = {"workload": f"the same workload at {wa['grid']}^2, {sum(wa['n'])} particles, "
+                               f"{accuracy['steps']} steps (set-up included)",
+                   "seconds": el,
+                   "particle_steps_per_s": sum(wa["n"]) * accuracy["steps"] / el,
+                   "cores": threads}
+        log(f"oracle accuracy run: {el:.1f} s")
     tf, tp, ns = oracle_sample(po, w, frac, steps)
     t_full = tf + frac * tp
     ntot = sum(w["n"])
@@ -168,6 +212,8 @@ def cpu_baseline(w, frac=8, steps=10):
     }
     if single:
         out["single_core"] = single
+    if acc_run:
+        out["accuracy_reference_run"] = acc_run
     # the reference's own best single-node figure for this grid, as published
     # (perf/strong-scaling-fixed/csv/time.csv:2): not re-measured here
     out["published_reference"] = {
@@ -176,6 +222,69 @@ def cpu_baseline(w, frac=8, steps=10):
         "hardware": "MareNostrum 4, 1 node, 48 cores",
         "source": "perf/strong-scaling-fixed/csv/time.csv:2"}
     return out
+
+
+def fp32_oracle_leg_gpu(ca, grid, steps, seed):
+    """The FP32 run of the workload scaled to grid^2 (same particles per
+    cell, species and constants; the full run's tile shape and particles per
+    thread): its field-energy history over `steps` steps.  The oracle's
+    history of the same input is computed in the CPU-baseline leg."""
+    from cpic_amd import autotune
+    w = workload(grid, 64)
+    tile, ppt = autotune.tile_for(2048, 32), autotune.ppt_for(2048, 32)
+    d = make_device(ca, w, 32, 0, seed=seed, tile=tile, ppt=ppt)
+    d.pre_step()
+    d.run(0, steps)
+    d.sync()
+    hist = d.energy(maxn=steps)["field"][:steps]
+    d.close()
+    return dict(w=w, seed=seed, steps=steps, gpu=np.array(hist), tile=list(tile), ppt=ppt)
+
+
+def acc_report(acc):
+    """FP32 field-energy error against the oracle, per step."""
+    g, o = acc["gpu"], acc.get("oracle")
+    if o is None:
+        return None
+    n = min(len(g), len(o))
+    rel = np.abs(g[:n] - o[:n]) / np.abs(o[:n])
+    above = np.nonzero(rel > 1e-5)[0]
+    w = acc["w"]
+    return {"steps": int(n), "reference": "FP64 oracle (CPU) on the same seeded input",
+            "workload": f"two-stream {w['grid']}^2, 64 ppc, {sum(w['n'])} particles, "
+                        f"electrons + ions (1:1836), FP32 particles + FP64 FFT Poisson",
+            "tile": acc["tile"], "ppt": acc["ppt"],
+            "max_rel": float(rel.max()), "max_rel_first_100": float(rel[:100].max()),
+            "first_step_above_1e-5": int(above[0]) if len(above) else None,
+            "energy_growth": float(o[n - 1] / o[0])}
+
+
+def stream_ceiling(total_bytes, achieved):
+    """Best rate of a hand-written read+write stream (lib/libcpic_probe.so:
+    16 B per element, dwordx4 loads and stores, 1-8 loads in flight per
+    thread) moving the push's algorithmic bytes, half read and half written."""
+    import ctypes
+    try:
+        probe = ctypes.CDLL(os.path.join(ROOT, "cpic_amd", "lib", "libcpic_probe.so"))
+    except OSError as e:
+        return {"ceiling_GBs": None, "ceiling_error": str(e)}
+    probe.cpic_probe_stream.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
+    half = total_bytes // 2 // 16 * 16
+    rates = {}
+    for u in (1, 2, 4, 8):
+        ms = ctypes.c_double()
+        if probe.cpic_probe_stream(half, u, 10, ctypes.byref(ms)) == 0 and ms.value > 0:
+            rates[str(u)] = 2 * half / (ms.value / 1e3) / 1e9
+    if not rates:
+        return {"ceiling_GBs": None}
+    best = max(rates.values())
+    log(f"stream ceiling: {rates}")
+    return {"ceiling_GBs": best,
+            "ceiling_kernel": (f"read {half} B + write {half} B, 16 B per element (dwordx4), "
+                               f"best of 1/2/4/8 loads in flight per thread"),
+            "ceiling_GBs_by_loads_in_flight": rates,
+            "frac_of_ceiling": achieved / best if achieved else None}
 
 
 def traffic_from_profiles(precision, name=None):
@@ -210,52 +319,15 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     log(f"copy calibration: {reps} x {nbytes} bytes device to device")
 
 
-ENSEMBLE_GRIDS = (32, 64, 128, 256)
-
-
-def ensemble_workload(nplasmas=256, seed=256):
-    """Per plasma: grid edge, particles per cell, drift v0 and spread sigma
-    (unit cells, L = grid, m = ppc so that w_p = 1, dt = 0.1)."""
-    rng = np.random.default_rng(seed)
-    grids = rng.choice(ENSEMBLE_GRIDS, nplasmas)
-    ppcs = rng.integers(16, 129, nplasmas)
-    v0 = np.linspace(0.05, 0.2, nplasmas)
-    sigma = np.linspace(0.005, 0.02, nplasmas)
-    return [dict(grid=int(g), ppc=int(c), v0=float(v), sigma=float(s), dt=0.1)
-            for g, c, v, s in zip(grids, ppcs, v0, sigma)]
-
-
 def run_ensemble(args, ca):
-    from cpic_amd import autotune
-    plist = ensemble_workload(args.plasmas)
+    """The batched ensemble (cpic_amd.ensemble): returns its JSON object."""
+    from cpic_amd import autotune, ensemble
+    plist = ensemble.workload(args.plasmas)
     table = autotune.load()
-    missing = sorted({p["grid"] for p in plist if f"fp32/{p['grid']}" not in table})
-    if missing and args.autotune:
-        table = autotune.tune(missing, 32, log=log)
-    # one particles-per-thread setting per batch: that of the grid edge
-    # carrying the most particles
-    load = {}
-    for p in plist:
-        load[p["grid"]] = load.get(p["grid"], 0) + p["grid"] ** 2 * p["ppc"]
-    ppt = autotune.ppt_for(max(load, key=load.get), 32, table)
-    descs = []
-    for p in plist:
-        g = p["grid"]
-        descs.append(ca.make_desc(g, g, (float(g), float(g)), p["dt"], [-1.0, -1.0],
-                                  [float(p["ppc"]), float(p["ppc"])], precision=32,
-                                  tile=autotune.tile_for(g, 32, table), flags=0, ppt=ppt))
     t0 = time.time()
-    b = ca.Batch(descs)
-    empty = np.empty(0, np.int64)
-    ntot = 0
-    for i, p in enumerate(plist):
-        rng = np.random.default_rng([args.seed, i])
-        parts, _ = autotune.two_stream(rng, p["grid"], p["ppc"], p["v0"], p["sigma"])
-        for s, (x, y, ux, uy) in enumerate(parts):
-            b.upload(s, empty, x, y, ux, uy, plasma=i)
-            ntot += len(x)
-        if i % 32 == 31:
-            log(f"ensemble: generated {i + 1} plasmas, {ntot} particles")
+    ppt = ensemble.batch_ppt(plist, table)
+    b = ca.Batch(ensemble.descs(plist, table, ppt=ppt))
+    ntot = ensemble.upload(b, plist, args.seed, log=log)
     b.pre_step()
     b.run(0, args.warmup)
     b.sync()
@@ -278,7 +350,7 @@ def run_ensemble(args, ca):
     alg = ntot * 32 + nodes * 24
     achieved = alg / avg_push / 1e9 if avg_push > 0 else None
     value = ntot * steps / el
-    grids = {g: sum(1 for p in plist if p["grid"] == g) for g in ENSEMBLE_GRIDS}
+    grids = {g: sum(1 for p in plist if p["grid"] == g) for g in ensemble.GRIDS}
     out = {
         "metric": "particle-steps/s, batched ensemble of independent two-stream plasmas",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -290,7 +362,8 @@ def run_ensemble(args, ca):
                                f"{ntot} particles, FP32 particles + FP64 FFT Poisson, "
                                f"captured-graph replay",
                    "plasmas": len(plist), "particles": ntot, "nodes": nodes,
-                   "tiles": {str(g): list(autotune.tile_for(g, 32, table)) for g in ENSEMBLE_GRIDS},
+                   "tiles": {str(g): list(autotune.tile_for(g, 32, table))
+                             for g in ensemble.GRIDS},
                    "ppt": ppt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -301,7 +374,7 @@ def run_ensemble(args, ca):
         "kernels_ms": {k: ms / max(n, 1) for k, (n, ms) in kt.items()},
     }
     b.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_adjoint(args):
@@ -369,7 +442,7 @@ def run_adjoint(args):
             "particle_steps_per_s": ps / (t2 - t0)},
         "loss": loss.value,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -394,8 +467,14 @@ def main():
     ap.add_argument("--plasmas", type=int, default=256, help="ensemble: number of plasmas")
     ap.add_argument("--ens-steps", type=int, default=300, help="ensemble: steps incl. warmup")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--autotune", action="store_true",
-                    help="ensemble: tune tile shapes of grids missing from the cache first")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="default workload: skip the ensemble object")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="default workload: skip the adjoint object")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the HBM streaming-ceiling probe")
+    ap.add_argument("--oracle-grid", type=int, default=256,
+                    help="grid edge of the FP32-vs-oracle accuracy leg (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -409,10 +488,10 @@ def main():
     if args.calibrate_copy:
         copy_calibration()
     if args.workload == "ensemble":
-        run_ensemble(args, ca)
+        print(json.dumps(run_ensemble(args, ca)), flush=True)
         return
     if args.workload == "adjoint":
-        run_adjoint(args)
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     w = workload(args.grid, args.ppc)
     ntot = sum(w["n"])
@@ -454,19 +533,32 @@ def main():
             log(f"   {k:24s} {n:6d} launches {ms / max(n, 1):9.4f} ms avg")
         hist = None
         snaps = {}
+        late = None
         if args.accuracy_steps > 0:
-            # untimed, same seed in both precisions: rho and E snapshots at
-            # fixed iterations, the field-energy history up to accuracy_steps
+            # same seed in both precisions: rho and E snapshots at fixed
+            # iterations, the field-energy history up to accuracy_steps; the
+            # last 100 steps are timed on the production path (late window:
+            # after many migrations, not right after the upload's sort)
+            d.set_profiling(False)
             done = args.warmup + 2 * steps
             for at in sorted({max(done, 200), args.accuracy_steps}):
                 if at > done:
-                    d.run(done, at - done)
+                    lw = min(100, at - done) if at == args.accuracy_steps else 0
+                    if at - lw > done:
+                        d.run(done, at - lw - done)
+                    d.sync()
+                    if lw:
+                        t0 = time.perf_counter()
+                        d.run(at - lw, lw)
+                        d.sync()
+                        late = dict(first=at - lw, steps=lw,
+                                    ms=(time.perf_counter() - t0) / lw * 1e3)
                     done = at
                 d.sync()
                 snaps[done] = {f: d.field(f) for f in ("rho", "Ex")}
             hist = d.energy()["field"][:args.accuracy_steps]
         res[prec] = dict(elapsed=el, elapsed_prof=el_prof, steps=steps, kt=kt, stats=st,
-                         hist=hist, snaps=snaps, ppt=ppt or 2)
+                         hist=hist, snaps=snaps, ppt=ppt or 2, late=late)
         d.close()
         del d
 
@@ -486,6 +578,9 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": r32["elapsed"] / r32["steps"] * 1e3,
         "ms_per_step_with_kernel_timers": r32["elapsed_prof"] / r32["steps"] * 1e3,
+        "ms_per_step_late": r32["late"]["ms"] if r32["late"] else None,
+        "late_window": (f"steps {r32['late']['first']}-{r32['late']['first'] + r32['late']['steps']}"
+                        f" of the accuracy leg, graph replay") if r32["late"] else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -510,6 +605,7 @@ def main():
         n64, ms64 = r64["kt"].get("push_gather_deposit", (0, 0.0))
         a64 = ms64 / max(n64, 1) / 1e3
         out["fp64"] = {"value": v64, "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
+                       "ms_per_step_late": r64["late"]["ms"] if r64["late"] else None,
                        "push_achieved_GBs": (ntot * 64 + ng * 24) / a64 / 1e9 if a64 else None,
                        "tile": r64["stats"]["tile"], "ppt": r64["ppt"]}
         out["fp32_over_fp64"] = value / v64
@@ -536,11 +632,31 @@ def main():
                 "Ex_rel_l2": {str(k): rl2(res[32]["snaps"][k]["Ex"], r64["snaps"][k]["Ex"])
                               for k in common},
             }
+    if not args.no_ceiling:
+        out["roofline"].update(stream_ceiling(alg_bytes, achieved))
+    acc = None
+    if args.oracle_grid > 0 and args.accuracy_steps > 0:
+        acc = fp32_oracle_leg_gpu(ca, args.oracle_grid, args.accuracy_steps, args.seed)
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(w)
+            out["cpu_baseline"] = cpu_baseline(w, accuracy=acc)
         except Exception as e:  # the baseline must not hide the GPU result
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        if acc is not None:
+            out["fp32_field_energy_error_vs_oracle"] = acc_report(acc)
+    if not args.no_ensemble:
+        e = run_ensemble(args, ca)
+        out["ensemble"] = {k: e[k] for k in ("value", "unit", "ms_per_step", "steps", "dtype")}
+        out["ensemble"].update(workload=e["config"]["workload"],
+                               push_frac=e["roofline"]["frac"],
+                               push_avg_launch_ms=e["roofline"]["avg_launch_ms"])
+    if not args.no_adjoint:
+        a = run_adjoint(args)
+        out["adjoint"] = {k: a[k] for k in ("value", "unit", "ms_per_step", "steps", "dtype",
+                                            "forward_device_s", "backward_device_s")}
+        out["adjoint"].update(workload=a["config"]["workload"],
+                              wall_clock_particle_steps_per_s=a["wall_clock_incl_host_transfers"][
+                                  "particle_steps_per_s"])
     print(json.dumps(out), flush=True)
 
 

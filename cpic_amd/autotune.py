@@ -1,18 +1,26 @@
-"""Tile-shape and particles-per-thread autotuner for batched runs.
+"""Tile-shape and particles-per-thread autotuner.
 
-For each grid edge and precision, times the full step (solve + push +
-migration + deposit, captured-graph replay) of a batch of identical
-two-stream plasmas under every candidate (tile shape, particles per thread
-and chunk of the push kernel) and keeps the fastest.
+Two modes, both timing the full step (solve + push + migration + deposit,
+captured-graph replay) and keeping the fastest candidate (tile shape, and
+particles per thread and chunk of the push kernel):
+
+- `tune`: per grid edge and precision, a batch of identical two-stream
+  plasmas (one plasma at 2048^2): the single-run workloads.
+- `tune_ensemble`: the batched-ensemble mix itself (cpic_amd.ensemble): the
+  tile of each grid edge is tuned inside the mixed batch, one edge at a
+  time with the others at their current choice, then the batch's particles
+  per thread.
+
 Results are cached as JSON next to this file (tuned/tiles.json), keyed by
-"fp<precision>/<grid edge>", with the measured rates, so a later run (or
-another process) reuses them without timing again.
+"fp<precision>/<grid edge>", with the measured rates and what they were
+measured on ("tuned_on"), so a later run (or another process) reuses them.
 
 The reference fixes its blocking by configuration (`simulation.plasma_chunks`,
 `simulation.pblock_nmax`, src/sim.c:62-66); here the tile edge is the
 analogous knob, and it is chosen by measurement per grid size.
 
-    python -m cpic_amd.autotune --grids 32 64 128 256 --precision 32
+    python -m cpic_amd.autotune --grids 2048 --precision 32
+    python -m cpic_amd.autotune --ensemble
 """
 import argparse
 import json
@@ -110,6 +118,71 @@ def tune(grids, precision=32, ppc=64, target_particles=1 << 25, steps=20, warmup
         table[f"fp{precision}/{grid}"] = {
             "tile": [int(v) for v in shape.split("x")], "ppt": int(ppt), "ppc": ppc,
             "plasmas": nplasmas, "rates": rates,
+            "tuned_on": f"{nplasmas} identical {grid}^2 two-stream plasmas at {ppc} ppc",
+        }
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    return table
+
+
+def ensemble_sample(every=4):
+    """The ensemble mix the tuner measures: every `every`-th plasma of the
+    256-plasma workload (the same grid, density, v0 and sigma draws)."""
+    from cpic_amd import ensemble
+    return ensemble.workload(256)[::every]
+
+
+def tune_ensemble(plist=None, seed=2024, steps=20, warmup=6, candidates=CANDIDATES,
+                  path=CACHE, log=print):
+    """Tunes fp32 tiles per grid edge and the batch's particles per thread on
+    the ensemble mix (plist, default ensemble_sample()), and writes them to
+    the table with the rates and the mix they were measured on."""
+    from cpic_amd import ensemble
+    plist = ensemble_sample() if plist is None else plist
+    table = load(path)
+    parts = [ensemble.particles(p, seed, i) for i, p in enumerate(plist)]
+    ntot = sum(len(x) for pp in parts for x, _, _, _ in pp)
+    edges = sorted({p["grid"] for p in plist},
+                   key=lambda g: -sum(p["grid"] ** 2 * p["ppc"] for p in plist if p["grid"] == g))
+    current = {g: tuple(tile_for(g, 32, table)) for g in edges}
+    ppt = ensemble.batch_ppt(plist, table)
+
+    def rate(tiles, ppt_):
+        b = ca.Batch(ensemble.descs(plist, table, flags=ca.F_NO_DIAG, ppt=ppt_, tiles=tiles))
+        ensemble.upload(b, plist, seed, parts=parts)
+        b.pre_step()
+        b.run(0, warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(warmup, steps)
+        b.sync()
+        el = time.perf_counter() - t0
+        b.close()
+        return ntot * steps / el
+
+    rates = {g: {} for g in edges}
+    for g in edges:
+        for tile in candidates:
+            if tile[0] > g or tile[1] > g:
+                continue
+            r = rate({**current, g: tile}, ppt)
+            rates[g][f"{tile[0]}x{tile[1]}/{ppt}"] = r
+            log(f"autotune ensemble: edge {g} tile {tile} ppt {ppt}: {r:.4g} particle-steps/s")
+        best = max(rates[g], key=rates[g].get)
+        current[g] = tuple(int(v) for v in best.split("/")[0].split("x"))
+    ppt_rates = {}
+    for p in PPT:
+        ppt_rates[p] = rate(current, p)
+        log(f"autotune ensemble: tiles {current} ppt {p}: {ppt_rates[p]:.4g} particle-steps/s")
+    ppt = max(ppt_rates, key=ppt_rates.get)
+    mix = (f"ensemble mix: {len(plist)} plasmas (every 4th of the 256-plasma workload), "
+           f"{ntot} particles, grids {sorted(edges)}")
+    for g in edges:
+        table[f"fp32/{g}"] = {
+            "tile": list(current[g]), "ppt": int(ppt), "plasmas": len(plist),
+            "rates": rates[g], "ppt_rates": {str(k): v for k, v in ppt_rates.items()},
+            "tuned_on": mix,
         }
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
@@ -123,9 +196,14 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--out", default=CACHE)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="tune fp32 tiles and particles per thread on the ensemble mix")
     args = ap.parse_args()
-    tune(args.grids, args.precision, args.ppc, path=args.out,
-         log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    lg = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
+    if args.ensemble:
+        tune_ensemble(path=args.out, log=lg)
+    else:
+        tune(args.grids, args.precision, args.ppc, path=args.out, log=lg)
     print(json.dumps(load(args.out)))
 
 

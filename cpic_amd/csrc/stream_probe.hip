@@ -1,0 +1,91 @@
+// HBM streaming ceiling of the device, for the measurement next to the
+// push's achieved bandwidth: a kernel that reads one buffer and writes
+// another, 16 B per element (one dwordx4 load and one dwordx4 store), with
+// U loads in flight per thread before its first store -- the access shape
+// of the push's particle stream (state read once, written once) without its
+// arithmetic, LDS work or atomics.  Not part of the PIC library's C ABI:
+// built as lib/libcpic_probe.so and loaded only by bench.py.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#define PROBE_BLOCK 256
+
+template <int U>
+__global__ __launch_bounds__(PROBE_BLOCK)
+void k_stream(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n)
+{
+	int64_t base = (int64_t) blockIdx.x * U * PROBE_BLOCK + threadIdx.x;
+	float4 v[U];
+#pragma unroll
+	for(int u = 0; u < U; u++)
+	{
+		int64_t i = base + (int64_t) u * PROBE_BLOCK;
+		if(i < n)
+			v[u] = a[i];
+	}
+#pragma unroll
+	for(int u = 0; u < U; u++)
+	{
+		int64_t i = base + (int64_t) u * PROBE_BLOCK;
+		if(i < n)
+			b[i] = v[u];
+	}
+}
+
+template <int U>
+static void launch(const float4 *a, float4 *b, int64_t n, hipStream_t s)
+{
+	int64_t blocks = (n + (int64_t) U * PROBE_BLOCK - 1) / ((int64_t) U * PROBE_BLOCK);
+	hipLaunchKernelGGL(k_stream<U>, dim3((unsigned) blocks), dim3(PROBE_BLOCK), 0, s, a, b, n);
+}
+
+// Average milliseconds per launch over `reps` launches (after one warm-up)
+// of the U = `unroll` kernel over two buffers of `bytes` bytes each (HIP
+// events on a stream of its own).  Returns 0, or -1 on a bad argument or a
+// HIP error (named on stderr).
+extern "C" int
+cpic_probe_stream(int64_t bytes, int unroll, int reps, double *ms_avg)
+{
+	if(bytes < 16 || reps < 1 || !ms_avg || (unroll != 1 && unroll != 2 && unroll != 4 &&
+				unroll != 8))
+		return -1;
+	int64_t n = bytes / 16;
+	float4 *a = NULL, *b = NULL;
+	hipStream_t s = NULL;
+	hipEvent_t e0 = NULL, e1 = NULL;
+	float ms = 0.0f;
+	int ret = -1;
+	if(hipMalloc(&a, 16 * (size_t) n) != hipSuccess || hipMalloc(&b, 16 * (size_t) n) != hipSuccess ||
+			hipMemset(a, 1, 16 * (size_t) n) != hipSuccess ||
+			hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+			hipEventCreate(&e1) != hipSuccess)
+		goto out;
+	for(int r = 0; r <= reps; r++)
+	{
+		if(r == 1 && hipEventRecord(e0, s) != hipSuccess)
+			goto out;
+		switch(unroll)
+		{
+			case 1: launch<1>(a, b, n, s); break;
+			case 2: launch<2>(a, b, n, s); break;
+			case 4: launch<4>(a, b, n, s); break;
+			default: launch<8>(a, b, n, s); break;
+		}
+	}
+	if(hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+			hipEventSynchronize(e1) != hipSuccess ||
+			hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+		goto out;
+	*ms_avg = (double) ms / reps;
+	ret = 0;
+out:
+	if(ret)
+		fprintf(stderr, "cpic_probe_stream: HIP error: %s\n", hipGetErrorString(hipGetLastError()));
+	if(e0) hipEventDestroy(e0);
+	if(e1) hipEventDestroy(e1);
+	if(s) hipStreamDestroy(s);
+	hipFree(a);
+	hipFree(b);
+	return ret;
+}

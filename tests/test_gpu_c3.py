@@ -3,9 +3,10 @@ tile-relative particles, FP32 push, FP64 Poisson solve and diagnostics)
 against the FP64 oracle.
 
 - Scaled to 256^2 at 64 particles per cell (4,194,304 particles), with the
-  production shape of the full-size run (32x16 tiles, 4 particles per
-  thread): after 20 steps rho, E and the particle state are within 1e-5
-  relative L2 of the oracle's, and the field-energy history of 1000 steps
+  tile shape and particles per thread tuned for the full-size run (the
+  autotuner's 2048^2 entry): after 20 steps rho, E and the particle state
+  are within 1e-5 relative L2 of the oracle's, and the field-energy history
+  of 1000 steps
   is within 1e-5 (max relative error per step) of the oracle's history.
 - The full size, 2048^2 with 268,435,456 particles, for 50 steps replayed
   from captured graphs: invariants only (the oracle would need ~15 minutes
@@ -34,8 +35,10 @@ def test_c3_scaled_fp32_vs_fp64_oracle():
     t0 = time.time()
     grid, L = w["grid"], (w["L"], w["L"])
     o = po.Oracle(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), nchunks=64)
+    from cpic_amd import autotune
     d = ca.Device(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), precision=32,
-                  tile=(32, 16), ppt=4, flags=ca.F_IDS)
+                  tile=autotune.tile_for(2048, 32), ppt=autotune.ppt_for(2048, 32),
+                  flags=ca.F_IDS)
     for s in range(2):
         ids, x, y, ux, uy = c3_species(w, s, seed=2024)
         o.add(s, ids, x, y, ux, uy)
@@ -83,8 +86,9 @@ def test_c3_full_size_invariants():
     w = c3_workload(2048)
     t0 = time.time()
     grid, L = w["grid"], (w["L"], w["L"])
+    from cpic_amd import autotune
     d = ca.Device(grid, grid, L, w["dt"], list(w["q"]), list(w["m"]), precision=32,
-                  tile=(32, 16), ppt=4)
+                  tile=autotune.tile_for(2048, 32), ppt=autotune.ppt_for(2048, 32))
     for s in range(2):
         ids, x, y, ux, uy = c3_species(w, s, seed=2024, ids=False)
         d.upload(s, ids, x, y, ux, uy)
