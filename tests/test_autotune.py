@@ -26,9 +26,30 @@ def test_table_covers_bench_grids_with_valid_choices():
             assert pow2(tx) and pow2(ty) and tx <= min(grid, 64) and ty <= min(grid, 64), key
             assert e["ppt"] in autotune.PPT, key
             best = max(e["rates"], key=e["rates"].get)
-            assert best == f"{tx}x{ty}/{e['ppt']}", (key, best)
+            if "ppt_rates" in e:
+                # tuned inside the ensemble mix: tile first, then the batch's ppt
+                assert best.split("/")[0] == f"{tx}x{ty}", (key, best)
+                assert max(e["ppt_rates"], key=e["ppt_rates"].get) == str(e["ppt"]), key
+            else:
+                assert best == f"{tx}x{ty}/{e['ppt']}", (key, best)
             assert autotune.tile_for(grid, prec, table) == (tx, ty)
             assert autotune.ppt_for(grid, prec, table) == e["ppt"]
+
+
+def test_table_is_the_tuners_output_for_the_workloads_that_use_it():
+    """The ensemble's grids were tuned inside the ensemble mix (one edge at a
+    time, then the batch's particles per thread), the 2048^2 entries on the
+    single-plasma workload: each entry records what it was measured on."""
+    table = autotune.load()
+    for grid in (32, 64, 128, 256):
+        e = table[f"fp32/{grid}"]
+        assert e["tuned_on"].startswith("ensemble mix"), grid
+        assert "ppt_rates" in e
+    ppts = {table[f"fp32/{g}"]["ppt"] for g in (32, 64, 128, 256)}
+    assert len(ppts) == 1          # one particles-per-thread setting per batch
+    for prec in (32, 64):
+        e = table[f"fp{prec}/2048"]
+        assert e["tuned_on"].startswith("1 identical 2048^2"), prec
 
 
 def test_untuned_grid_uses_defaults():
