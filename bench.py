@@ -279,12 +279,25 @@ def stream_ceiling(total_bytes, achieved):
     if not rates:
         return {"ceiling_GBs": None}
     best = max(rates.values())
-    log(f"stream ceiling: {rates}")
-    return {"ceiling_GBs": best,
-            "ceiling_kernel": (f"read {half} B + write {half} B, 16 B per element (dwordx4), "
-                               f"best of 1/2/4/8 loads in flight per thread"),
-            "ceiling_GBs_by_loads_in_flight": rates,
-            "frac_of_ceiling": achieved / best if achieved else None}
+    out = {"ceiling_GBs": best,
+           "ceiling_kernel": (f"read {half} B + write {half} B, 16 B per element (dwordx4), "
+                              f"best of 1/2/4/8 loads in flight per thread"),
+           "ceiling_GBs_by_loads_in_flight": rates,
+           "frac_of_ceiling": achieved / best if achieved else None}
+    # the push's own load/store pattern without its arithmetic
+    probe.cpic_probe_stream_groups.argtypes = [ctypes.c_int64, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    if probe.cpic_probe_stream_groups(half, 10, ctypes.byref(ms)) == 0 and ms.value > 0:
+        moved = 2 * (half // 16 // 1024 * 1024 * 16)
+        pat = moved / (ms.value / 1e3) / 1e9
+        out.update(pattern_ceiling_GBs=pat,
+                   pattern_kernel=("the push's grouped FP32 state: dwordx4 loads of 4 slots per "
+                                   "thread and component, one dword store per slot and component, "
+                                   "no arithmetic"),
+                   frac_of_pattern_ceiling=achieved / pat if achieved else None)
+    log(f"stream ceiling: {rates}, push pattern: {out.get('pattern_ceiling_GBs')}")
+    return out
 
 
 def traffic_from_profiles(precision, name=None):

@@ -398,8 +398,8 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 		rows += g->ny / g->R;
 		cols += (g->nx / 2 + g->C - 1) / g->C;   // kx = 0 and nx/2 share a column
 		nblk += (n + EF_NODES - 1) / EF_NODES;
-		size_t lr = sizeof(double2) * (size_t) g->R * (size_t) hx;
-		size_t lc = sizeof(double2) * (size_t) g->C * (size_t) g->ny;
+		size_t lr = fft_lds_bytes((int64_t) g->R * hx);
+		size_t lc = fft_lds_bytes((int64_t) g->C * g->ny);
 		if(lr > lds_rows) lds_rows = lr;
 		if(lc > lds_cols) lds_cols = lc;
 		if(tiles > (1 << 30) || buckets > (1 << 30) || rows > (1 << 30) || cols > (1 << 30) ||

@@ -290,6 +290,11 @@ void cpic_kt_end(cpic_dev *d, int slot);
 #ifndef FFT_COL_LDS_SMALL
 #define FFT_COL_LDS_SMALL (32 * 1024)
 #endif
+// Bytes of an FFT LDS buffer holding n complex values.
+static inline size_t fft_lds_bytes(int64_t n)
+{
+	return 16 * (size_t) (n > 0 ? n : 1);
+}
 
 static inline int ilog2(int64_t n)
 {

@@ -160,15 +160,18 @@ static __device__ __forceinline__ void dft_small(double2 *v, double sign)
 #define FFT_MIN_WAVES 8
 static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
 
-// One Stockham stage of radix R over nbat transforms of length n = 2^lgn
-// (transform b at buf + b*stride), current span p = 2^lgp: butterfly i of a
-// transform reads x[i + t n/R], applies w^(t k) with w = exp(sign 2 pi i/(R p)),
-// k = i mod p, and writes the R-point DFT to y[(i - k) R + k + t p].  All
-// reads precede a barrier and all writes follow it, so one LDS buffer holds
-// both.  Needs (n/R)*nbat <= (FFT_MAXV/R) * NT.
-template <int R, int NT>
+// One Stockham stage of radix R over nbat transforms of length n = 2^lgn,
+// current span p = 2^lgp: butterfly i of a transform reads x[i + t n/R],
+// applies w^(t k) with w = exp(sign 2 pi i/(R p)), k = i mod p, and writes
+// the R-point DFT to y[(i - k) R + k + t p].  x comes from the LDS buffer
+// (transform b at b*stride, IN_LDS) or from in(b, j); y goes to the LDS
+// buffer (OUT_LDS) or to out(b, j, value).  In place in LDS, all reads
+// precede a barrier and all writes follow it, so one buffer holds both; a
+// stage that writes LDS ends with a barrier.  Needs (n/R)*nbat <=
+// (FFT_MAXV/R) * NT.
+template <int R, int NT, bool IN_LDS, bool OUT_LDS, class In, class Out>
 static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
-		int nbat, int lgp, const TwRef &tw, int lgN, double sign)
+		int nbat, int lgp, const TwRef &tw, int lgN, double sign, In in, Out out)
 {
 	constexpr int LGR = R == 8 ? 3 : R == 4 ? 2 : 1;
 	constexpr int MAXB = FFT_MAXV / R;
@@ -176,62 +179,127 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 	const int total = nb * nbat;
 	const int tsh = lgN - LGR - lgp;       // table index of w^(t k) = t k 2^tsh
 	double2 v[MAXB][R];
-	int jo[MAXB];
+	int jo[MAXB], bo[MAXB];
 #pragma unroll
 	for(int m = 0; m < MAXB; m++)
 	{
 		int t_ = threadIdx.x + m * NT;
 		jo[m] = -1;
+		bo[m] = 0;
 		if(t_ < total)
 		{
 			int bat = t_ >> lgnb, i = t_ & (nb - 1);
 			int k = i & (p - 1);
-			const double2 *src = buf + bat * stride + i;
 #pragma unroll
 			for(int r = 0; r < R; r++)
-				v[m][r] = src[r * nb];
+				v[m][r] = IN_LDS ? buf[bat * stride + i + r * nb] : in(bat, i + r * nb);
+			if(k)
+			{
 #pragma unroll
-			for(int r = 1; r < R; r++)
-				if(k)
+				for(int r = 1; r < R; r++)
 					v[m][r] = cmul(v[m][r], twiddle_at(tw, (r * k) << tsh, sign));
+			}
 			dft_small<R>(v[m], sign);
-			jo[m] = bat * stride + ((i - k) << LGR) + k;
+			jo[m] = ((i - k) << LGR) + k;
+			bo[m] = bat;
 		}
 	}
-	__syncthreads();
+	if(IN_LDS && OUT_LDS)
+		__syncthreads();
 #pragma unroll
 	for(int m = 0; m < MAXB; m++)
 		if(jo[m] >= 0)
 		{
 #pragma unroll
 			for(int r = 0; r < R; r++)
-				buf[jo[m] + r * p] = v[m][r];
+			{
+				if(OUT_LDS)
+					buf[bo[m] * stride + jo[m] + r * p] = v[m][r];
+				else
+					out(bo[m], jo[m] + r * p, v[m][r]);
+			}
 		}
-	__syncthreads();
+	if(OUT_LDS)
+		__syncthreads();
 }
 
-// FFT of nbat transforms of length n = 2^lgn at buf + b*stride, natural
-// order in and out (Stockham autosort: radix-8 stages after one radix-2 or
-// radix-4 stage for lg n mod 3), w = exp(sign 2 pi i/len).  Starts and ends
-// at a barrier.
-template <int NT>
-static __device__ void fft_block(double2 *buf, int stride, int lgn, int nbat,
-		const TwRef &tw, int lgN, double sign)
+struct NoIn {
+	__device__ double2 operator()(int, int) const { return make_double2(0.0, 0.0); }
+};
+struct NoOut {
+	__device__ void operator()(int, int, double2) const {}
+};
+
+// Radix of the first stage of a length-2^lgn transform: radix-8 stages after
+// one radix-2 or radix-4 stage for lg n mod 3.
+static __device__ __forceinline__ int first_lgr(int lgn)
 {
-	// radix-8 stages, after one radix-2 or radix-4 stage for lg n mod 3
-	int lgp = 0;
-	if(lgn % 3 == 1)
+	return lgn % 3 == 1 ? 1 : lgn % 3 == 2 ? 2 : 3;
+}
+
+// FFT of nbat transforms of length n = 2^lgn (Stockham autosort, natural
+// order in and out), w = exp(sign 2 pi i/len), reading element j of
+// transform b through in(b, j) and leaving the result in the LDS buffer
+// (transform b at b*stride).  Ends at a barrier.
+template <int NT, class In>
+static __device__ void fft_from(double2 *buf, int stride, int lgn, int nbat, const TwRef &tw,
+		int lgN, double sign, In in)
+{
+	NoOut none;
+	if(lgn == 0)
 	{
-		stockham_stage<2, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
-		lgp = 1;
+		for(int b = threadIdx.x; b < nbat; b += NT)
+			buf[b * stride] = in(b, 0);
+		__syncthreads();
+		return;
 	}
-	else if(lgn % 3 == 2)
-	{
-		stockham_stage<4, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
-		lgp = 2;
-	}
+	int lgp = first_lgr(lgn);
+	if(lgp == 1)
+		stockham_stage<2, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
+	else if(lgp == 2)
+		stockham_stage<4, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
+	else
+		stockham_stage<8, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
 	for(; lgp < lgn; lgp += 3)
-		stockham_stage<8, NT>(buf, stride, lgn, nbat, lgp, tw, lgN, sign);
+		stockham_stage<8, NT, true, true>(buf, stride, lgn, nbat, lgp, tw, lgN, sign,
+				NoIn(), none);
+}
+
+// The same transform from the LDS buffer (written before a barrier), its
+// result written element by element through out(b, j, value).
+template <int NT, class Out>
+static __device__ void fft_to(double2 *buf, int stride, int lgn, int nbat, const TwRef &tw,
+		int lgN, double sign, Out out)
+{
+	NoIn none;
+	if(lgn == 0)
+	{
+		for(int b = threadIdx.x; b < nbat; b += NT)
+			out(b, 0, buf[b * stride]);
+		return;
+	}
+	int l0 = first_lgr(lgn);
+	if(l0 == lgn)
+	{
+		if(l0 == 1)
+			stockham_stage<2, NT, true, false>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, out);
+		else if(l0 == 2)
+			stockham_stage<4, NT, true, false>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, out);
+		else
+			stockham_stage<8, NT, true, false>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, out);
+		return;
+	}
+	NoOut nout;
+	if(l0 == 1)
+		stockham_stage<2, NT, true, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, nout);
+	else if(l0 == 2)
+		stockham_stage<4, NT, true, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, nout);
+	else
+		stockham_stage<8, NT, true, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, none, nout);
+	int lgp = l0;
+	for(; lgp + 3 < lgn; lgp += 3)
+		stockham_stage<8, NT, true, true>(buf, stride, lgn, nbat, lgp, tw, lgN, sign, none, nout);
+	stockham_stage<8, NT, true, false>(buf, stride, lgn, nbat, lgp, tw, lgN, sign, none, out);
 }
 
 // Pass 1: R rows per block of NT threads; row y real of length nx = 2m.
@@ -253,53 +321,46 @@ void k_rows_r2c(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	__shared__ TwLds s_tw;
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
-	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
-	{
-		int r = i / m, j = i - r * m;
-		const double *row = rho + (int64_t) (y0 + r) * nx;
-		sh[r * stride + j] = make_double2(row[2 * j], row[2 * j + 1]);
-	}
-	__syncthreads();
-	if(lgm > 0)
-		fft_block<NT>(sh, stride, lgm, R, twr, lgN, -1.0);
+	// the first stage reads the rows straight from rho (as complex pairs)
+	const double2 *__restrict__ rows2 = (const double2 *) (rho + (int64_t) y0 * nx);
+	fft_from<NT>(sh, stride, lgm, R, twr, lgN, -1.0,
+			[=](int r, int j) { return rows2[(int64_t) r * m + j]; });
 
 	// X[k] = E + W^k O, X[m-k] = conj(E - W^k O), W = exp(-2 pi i/nx)
 	int npair = (m >> 1) + 1;
 	int tsh = lgN - (lgm + 1);
+	// fused with the transposed store ghat[k*ny + y]: consecutive threads
+	// take consecutive rows of one k, so each k writes R contiguous values
 	for(int i = threadIdx.x; i < R * npair; i += blockDim.x)
 	{
-		int r = i / npair, k = i - r * npair;
-		double2 *b = sh + r * stride;
+		int k = i / R, r = i - k * R;
+		const int b = r * stride;
+		double2 xk, xm;
 		if(k == 0)
 		{
-			double2 z = b[0];
-			b[0] = make_double2(z.x + z.y, 0.0);
-			b[m] = make_double2(z.x - z.y, 0.0);
+			double2 z = sh[b];
+			xk = make_double2(z.x + z.y, 0.0);
+			xm = make_double2(z.x - z.y, 0.0);
 		}
 		else if(2 * k == m)
 		{
-			double2 z = b[k];
-			b[k] = make_double2(z.x, -z.y);
+			double2 z = sh[b + k];
+			xk = xm = make_double2(z.x, -z.y);
 		}
 		else
 		{
-			double2 zk = b[k], zm = b[m - k];
+			double2 zk = sh[b + k], zm = sh[b + m - k];
 			// E = (zk + conj zm)/2, O = (zk - conj zm)/(2i)
 			double er = 0.5 * (zk.x + zm.x), ei = 0.5 * (zk.y - zm.y);
 			double dr = 0.5 * (zk.x - zm.x), di = 0.5 * (zk.y + zm.y);
 			double2 O = make_double2(di, -dr);
 			double2 wo = cmul(O, twiddle_at(twr, k << tsh, -1.0));
-			b[k] = make_double2(er + wo.x, ei + wo.y);
-			b[m - k] = make_double2(er - wo.x, -(ei - wo.y));
+			xk = make_double2(er + wo.x, ei + wo.y);
+			xm = make_double2(er - wo.x, -(ei - wo.y));
 		}
-	}
-	__syncthreads();
-
-	// transposed store: ghat[k*ny + y]
-	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
-	{
-		int k = i / R, r = i - k * R;
-		ghat[(int64_t) k * ny + y0 + r] = sh[r * stride + k];
+		ghat[(int64_t) k * ny + y0 + r] = xk;
+		if(2 * k != m)
+			ghat[(int64_t) (m - k) * ny + y0 + r] = xm;
 	}
 }
 
@@ -336,16 +397,11 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	__shared__ TwLds s_tw;
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
-	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-	{
-		if(packed && i < ny)
-			sh[i] = make_double2(ghat[i].x, ghat[(int64_t) m * ny + i].x);
-		else
-			sh[i] = ghat[(int64_t) k0 * ny + i];
-	}
-	__syncthreads();
-	if(lgny > 0)
-		fft_block<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0);
+	// the first stage reads the columns straight from ghat
+	fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, [=](int c, int y) {
+		return (packed && c == 0) ? make_double2(ghat[y].x, ghat[(int64_t) m * ny + y].x) :
+			ghat[(int64_t) (k0 + c) * ny + y];
+	});
 
 	// G exactly as MFT_init computes it (src/solver.c:269-280)
 	auto green = [&](int kx, int ky) {
@@ -410,20 +466,16 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		}
 	}
 	__syncthreads();
-	if(lgny > 0)
-		fft_block<NT>(sh, ny, lgny, ncol, twr, lgN, +1.0);
-
-	for(int i = threadIdx.x; i < ncol * ny; i += blockDim.x)
-	{
-		double2 v = sh[i];
-		if(packed && i < ny)
+	// the last stage writes the columns straight to ghat
+	fft_to<NT>(sh, ny, lgny, ncol, twr, lgN, +1.0, [=](int c, int y, double2 v) {
+		if(packed && c == 0)
 		{
-			ghat[i] = make_double2(v.x, 0.0);
-			ghat[(int64_t) m * ny + i] = make_double2(v.y, 0.0);
+			ghat[y] = make_double2(v.x, 0.0);
+			ghat[(int64_t) m * ny + y] = make_double2(v.y, 0.0);
 		}
 		else
-			ghat[(int64_t) k0 * ny + i] = v;
-	}
+			ghat[(int64_t) (k0 + c) * ny + y] = v;
+	});
 }
 
 // Pass 3: R rows per block, c2r along X (imaginary parts of X[0] and
@@ -447,61 +499,42 @@ void k_rows_c2r(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	__shared__ TwLds s_tw;
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
-	for(int i = threadIdx.x; i < R * (m + 1); i += blockDim.x)
-	{
-		int k = i / R, r = i - k * R;
-		sh[r * stride + k] = ghat[(int64_t) k * ny + y0 + r];
-	}
-	__syncthreads();
-
 	// Z[k] = A + iB, A = X[k] + conj X[m-k], B = (X[k] - conj X[m-k]) e^{+2 pi i k/nx}
 	int npair = (m >> 1) + 1;
 	int tsh = lgN - (lgm + 1);
+	// fused with the transposed load of X[k] and X[m-k] (consecutive
+	// threads take consecutive rows of one k); the twiddle factors are read
+	// right away, so their fill is published first
+	__syncthreads();
 	for(int i = threadIdx.x; i < R * npair; i += blockDim.x)
 	{
-		int r = i / npair, k = i - r * npair;
-		double2 *b = sh + r * stride;
+		int k = i / R, r = i - k * R;
+		const int b = r * stride;
+		double2 xk = ghat[(int64_t) k * ny + y0 + r];
+		double2 xm = ghat[(int64_t) (m - k) * ny + y0 + r];
 		if(k == 0)
-		{
-			double a0 = b[0].x, am = b[m].x;
-			b[0] = make_double2(a0 + am, a0 - am);
-		}
+			sh[b] = make_double2(xk.x + xm.x, xk.x - xm.x);
 		else if(2 * k == m)
-		{
-			// A = (2 Re x, 0), D = (0, 2 Im x), B = D e^{i pi/2} = (-2 Im x, 0),
-			// Z = A + iB
-			double2 x = b[k];
-			b[k] = make_double2(2.0 * x.x, -2.0 * x.y);
-		}
+			sh[b + k] = make_double2(2.0 * xk.x, -2.0 * xk.y);
 		else
 		{
-			double2 xk = b[k], xm = b[m - k];
 			double2 wk = twiddle_at(twr, k << tsh, 1.0);         // e^{+2 pi i k/nx}
 			double2 wm = twiddle_at(twr, (m - k) << tsh, 1.0);   // e^{+2 pi i (m-k)/nx}
-			// for k
 			double2 A = make_double2(xk.x + xm.x, xk.y - xm.y);
 			double2 D = make_double2(xk.x - xm.x, xk.y + xm.y);
 			double2 B = cmul(D, wk);
-			// for m-k
 			double2 A2 = make_double2(xm.x + xk.x, xm.y - xk.y);
 			double2 D2 = make_double2(xm.x - xk.x, xm.y + xk.y);
 			double2 B2 = cmul(D2, wm);
-			b[k] = make_double2(A.x - B.y, A.y + B.x);
-			b[m - k] = make_double2(A2.x - B2.y, A2.y + B2.x);
+			sh[b + k] = make_double2(A.x - B.y, A.y + B.x);
+			sh[b + m - k] = make_double2(A2.x - B2.y, A2.y + B2.x);
 		}
 	}
 	__syncthreads();
-	if(lgm > 0)
-		fft_block<NT>(sh, stride, lgm, R, twr, lgN, +1.0);
-
-	for(int i = threadIdx.x; i < R * m; i += blockDim.x)
-	{
-		int r = i / m, j = i - r * m;
-		double2 z = sh[r * stride + j];
-		double *row = phi + (int64_t) (y0 + r) * nx;
-		row[2 * j] = z.x;
-		row[2 * j + 1] = z.y;
-	}
+	// the last stage writes the rows straight to phi (as complex pairs)
+	double2 *__restrict__ prow = (double2 *) (phi + (int64_t) y0 * nx);
+	fft_to<NT>(sh, stride, lgm, R, twr, lgN, +1.0,
+			[=](int r, int j, double2 v) { prow[(int64_t) r * m + j] = v; });
 	// the same rows of the next rho buffer, which the coming push deposits
 	// into, and the next step's diagnostic slot start at zero
 	if(rho_zero_all)
@@ -706,6 +739,8 @@ cpic_fft_blocking(Geo *g, int nt_row, int nt_col)
 		R = g->ny;
 	while(r * 2 <= R)
 		r *= 2;
+	while(r > 1 && (int64_t) fft_lds_bytes(r * (g->nx / 2 + 1)) > lr)
+		r /= 2;
 	g->R = (int) r;
 	int64_t vc = nt_col == FFT_THREADS ? 4 * FFT_THREADS : (int64_t) FFT_MAXV * nt_col;
 	int64_t lc = nt_col == FFT_THREADS ? FFT_LDS_BYTES : FFT_COL_LDS_SMALL;
@@ -718,6 +753,8 @@ cpic_fft_blocking(Geo *g, int nt_row, int nt_col)
 		C = g->nx / 2;
 	if(C > 16)
 		C = 16;
+	while(C > 1 && (int64_t) fft_lds_bytes(C * g->ny) > lc)
+		C--;
 	g->C = (int) C;
 }
 
@@ -768,8 +805,8 @@ cpic_fft_blocking_ok(const Geo *g, int nt_row, int nt_col)
 	// dynamic LDS, leaving 16 KB for the kernels' static LDS (twiddle
 	// factors, reduction partials)
 	const int64_t lim = 144 * 1024;
-	return (int64_t) g->R * (g->nx / 2 + 1) * (int64_t) sizeof(double2) <= lim &&
-		(int64_t) g->C * g->ny * (int64_t) sizeof(double2) <= lim &&
+	return (int64_t) fft_lds_bytes((int64_t) g->R * (g->nx / 2 + 1)) <= lim &&
+		(int64_t) fft_lds_bytes((int64_t) g->C * g->ny) <= lim &&
 		(int64_t) g->R * (g->nx / 2) <= (int64_t) FFT_MAXV * nt_row &&
 		(int64_t) g->C * g->ny <= (int64_t) FFT_MAXV * nt_col;
 }
@@ -1026,8 +1063,8 @@ cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, vo
 	hipStream_t s = (hipStream_t) stream;
 	int lgN = ilog2(N);
 	int rows = (int) (ny / g.R), cols = (int) ((nx / 2 + g.C - 1) / g.C);
-	size_t lds_rows = sizeof(double2) * (size_t) g.R * (size_t) hx;
-	size_t lds_cols = sizeof(double2) * (size_t) g.C * (size_t) ny;
+	size_t lds_rows = fft_lds_bytes((int64_t) g.R * hx);
+	size_t lds_cols = fft_lds_bytes((int64_t) g.C * ny);
 	const Geo *geo = (const Geo *) (ws + WS_GEO);
 	const int *pre = (const int *) (ws + WS_PRE);
 	const StepState *st = (const StepState *) (ws + WS_STEP);

@@ -605,6 +605,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #endif
 
 
+
 // Leavers of one bucket wait in LDS and are placed in the leaver buffer with
 // one global atomic per bucket: a counter shared by every workgroup, hit once
 // per wavefront with a leaver, serialises in one L2 channel and stalls the
@@ -778,18 +779,31 @@ void k_push(PushArgs a)
 					}
 				}
 			}
-			// stayers: one LDS atomic per wavefront claims their slots in
-			// the other buffer (order within the bucket is not kept)
+			// stayers: one LDS atomic per wavefront and chunk claims their
+			// slots in the other buffer for all U rounds (order within the
+			// bucket is not kept)
+			unsigned long long smks[U];
+			int wbase = 0, wtot = 0;
 #pragma unroll
 			for(int u = 0; u < U; u++)
 			{
-				unsigned long long smk = __ballot(act[u] && !leave[u]);
+				smks[u] = __ballot(act[u] && !leave[u]);
+				wtot += __popcll(smks[u]);
+			}
+			if(wtot)
+			{
+				if(lane == 0)
+					wbase = atomicAdd(&s_w, wtot);
+				wbase = __builtin_amdgcn_readfirstlane(wbase);
+			}
+#pragma unroll
+			for(int u = 0; u < U; u++)
+			{
+				unsigned long long smk = smks[u];
+				int wb = wbase;
+				wbase += __popcll(smk);
 				if(smk)
 				{
-					int wb = 0;
-					if(lane == 0)
-						wb = atomicAdd(&s_w, __popcll(smk));
-					wb = __builtin_amdgcn_readfirstlane(wb);
 					if(act[u] && !leave[u])
 					{
 						int jj = wb + __popcll(smk & lt);

@@ -40,6 +40,83 @@ static void launch(const float4 *a, float4 *b, int64_t n, hipStream_t s)
 	hipLaunchKernelGGL(k_stream<U>, dim3((unsigned) blocks), dim3(PROBE_BLOCK), 0, s, a, b, n);
 }
 
+// The push's own access pattern without its arithmetic, LDS work or
+// atomics: FP32 particle state in groups of 64 slots (x[64] y[64] ux[64]
+// uy[64], 1 KiB per group), 1024 slots per workgroup.  Thread t loads slots
+// 4t..4t+3 of its chunk with one dwordx4 load per component, then writes
+// its u-th particle to output slot u*256 + t with one dword store per
+// component (consecutive lanes write consecutive slots, as the push's
+// stayer stores do).
+__global__ __launch_bounds__(PROBE_BLOCK)
+void k_stream_groups(const float *__restrict__ a, float *__restrict__ b, int64_t nslots)
+{
+	int64_t base = (int64_t) blockIdx.x * 1024;
+	int t = threadIdx.x;
+	int64_t j0 = base + 4 * t;
+	if(j0 + 4 > nslots)
+		return;
+	// element of slot s, component c: (s >> 6) * 256 + c * 64 + (s & 63)
+	const float4 *r = (const float4 *) (a + (j0 >> 6) * 256 + (j0 & 63));
+	float4 x = r[0], y = r[16], ux = r[32], uy = r[48];
+	float vx[4] = {x.x, x.y, x.z, x.w}, vy[4] = {y.x, y.y, y.z, y.w};
+	float vux[4] = {ux.x, ux.y, ux.z, ux.w}, vuy[4] = {uy.x, uy.y, uy.z, uy.w};
+#pragma unroll
+	for(int u = 0; u < 4; u++)
+	{
+		int64_t s = base + u * PROBE_BLOCK + t;
+		float *w = b + (s >> 6) * 256 + (s & 63);
+		w[0] = vx[u];
+		w[64] = vy[u];
+		w[128] = vux[u];
+		w[192] = vuy[u];
+	}
+}
+
+// Average milliseconds per launch of k_stream_groups over two buffers of
+// `bytes` bytes each (whole chunks of 1024 slots; 16 B per slot), as for
+// cpic_probe_stream.
+extern "C" int
+cpic_probe_stream_groups(int64_t bytes, int reps, double *ms_avg)
+{
+	if(bytes < 16384 || reps < 1 || !ms_avg)
+		return -1;
+	int64_t nslots = bytes / 16 / 1024 * 1024;
+	float *a = NULL, *b = NULL;
+	hipStream_t s = NULL;
+	hipEvent_t e0 = NULL, e1 = NULL;
+	float ms = 0.0f;
+	int ret = -1;
+	if(hipMalloc(&a, 16 * (size_t) nslots) != hipSuccess ||
+			hipMalloc(&b, 16 * (size_t) nslots) != hipSuccess ||
+			hipMemset(a, 1, 16 * (size_t) nslots) != hipSuccess ||
+			hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+			hipEventCreate(&e1) != hipSuccess)
+		goto out;
+	for(int r = 0; r <= reps; r++)
+	{
+		if(r == 1 && hipEventRecord(e0, s) != hipSuccess)
+			goto out;
+		hipLaunchKernelGGL(k_stream_groups, dim3((unsigned) (nslots / 1024)), dim3(PROBE_BLOCK),
+				0, s, a, b, nslots);
+	}
+	if(hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+			hipEventSynchronize(e1) != hipSuccess ||
+			hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+		goto out;
+	*ms_avg = (double) ms / reps;
+	ret = 0;
+out:
+	if(ret)
+		fprintf(stderr, "cpic_probe_stream_groups: HIP error: %s\n",
+				hipGetErrorString(hipGetLastError()));
+	if(e0) hipEventDestroy(e0);
+	if(e1) hipEventDestroy(e1);
+	if(s) hipStreamDestroy(s);
+	hipFree(a);
+	hipFree(b);
+	return ret;
+}
+
 // Average milliseconds per launch over `reps` launches (after one warm-up)
 // of the U = `unroll` kernel over two buffers of `bytes` bytes each (HIP
 // events on a stream of its own).  Returns 0, or -1 on a bad argument or a
