@@ -14,7 +14,11 @@
  * Errors: every call returns 0 or a negative CPIC_E* code (the reference
  * aborts instead, src/log.h:69-70).  Kernel-side conditions (|u| > umax,
  * non-finite values) are latched on the device and returned by the next
- * call that synchronises (cpic_dev_sync, downloads, cpic_dev_energy).
+ * call that synchronises (cpic_dev_sync, downloads, cpic_dev_energy).  A
+ * particle re-sort that fails part way (CPIC_ENOMEM or CPIC_EHIP from an
+ * upload's commit, cpic_dev_rebuild or cpic_dev_sync) leaves the handle
+ * unusable: every later call on it but cpic_dev_destroy returns
+ * CPIC_ESTATE.
  */
 #ifndef CPIC_HIP_H
 #define CPIC_HIP_H
