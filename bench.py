@@ -285,10 +285,10 @@ def stream_ceiling(total_bytes, achieved):
            "ceiling_GBs_by_loads_in_flight": rates,
            "frac_of_ceiling": achieved / best if achieved else None}
     # the push's own load/store pattern without its arithmetic
-    probe.cpic_probe_stream_groups.argtypes = [ctypes.c_int64, ctypes.c_int,
+    probe.cpic_probe_stream_groups.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                                ctypes.POINTER(ctypes.c_double)]
     ms = ctypes.c_double()
-    if probe.cpic_probe_stream_groups(half, 10, ctypes.byref(ms)) == 0 and ms.value > 0:
+    if probe.cpic_probe_stream_groups(half, 0, 10, ctypes.byref(ms)) == 0 and ms.value > 0:
         moved = 2 * (half // 16 // 1024 * 1024 * 16)
         pat = moved / (ms.value / 1e3) / 1e9
         out.update(pattern_ceiling_GBs=pat,

@@ -47,6 +47,7 @@ static void launch(const float4 *a, float4 *b, int64_t n, hipStream_t s)
 // its u-th particle to output slot u*256 + t with one dword store per
 // component (consecutive lanes write consecutive slots, as the push's
 // stayer stores do).
+template <bool VSTORE>
 __global__ __launch_bounds__(PROBE_BLOCK)
 void k_stream_groups(const float *__restrict__ a, float *__restrict__ b, int64_t nslots)
 {
@@ -58,6 +59,16 @@ void k_stream_groups(const float *__restrict__ a, float *__restrict__ b, int64_t
 	// element of slot s, component c: (s >> 6) * 256 + c * 64 + (s & 63)
 	const float4 *r = (const float4 *) (a + (j0 >> 6) * 256 + (j0 & 63));
 	float4 x = r[0], y = r[16], ux = r[32], uy = r[48];
+	if(VSTORE)
+	{
+		// the same 4 slots written back with one dwordx4 store per component
+		float4 *w = (float4 *) (b + (j0 >> 6) * 256 + (j0 & 63));
+		w[0] = x;
+		w[16] = y;
+		w[32] = ux;
+		w[48] = uy;
+		return;
+	}
 	float vx[4] = {x.x, x.y, x.z, x.w}, vy[4] = {y.x, y.y, y.z, y.w};
 	float vux[4] = {ux.x, ux.y, ux.z, ux.w}, vuy[4] = {uy.x, uy.y, uy.z, uy.w};
 #pragma unroll
@@ -74,9 +85,10 @@ void k_stream_groups(const float *__restrict__ a, float *__restrict__ b, int64_t
 
 // Average milliseconds per launch of k_stream_groups over two buffers of
 // `bytes` bytes each (whole chunks of 1024 slots; 16 B per slot), as for
-// cpic_probe_stream.
+// cpic_probe_stream; vstore = 1 writes with dwordx4 stores instead of the
+// push's per-slot stores.
 extern "C" int
-cpic_probe_stream_groups(int64_t bytes, int reps, double *ms_avg)
+cpic_probe_stream_groups(int64_t bytes, int vstore, int reps, double *ms_avg)
 {
 	if(bytes < 16384 || reps < 1 || !ms_avg)
 		return -1;
@@ -96,8 +108,12 @@ cpic_probe_stream_groups(int64_t bytes, int reps, double *ms_avg)
 	{
 		if(r == 1 && hipEventRecord(e0, s) != hipSuccess)
 			goto out;
-		hipLaunchKernelGGL(k_stream_groups, dim3((unsigned) (nslots / 1024)), dim3(PROBE_BLOCK),
-				0, s, a, b, nslots);
+		if(vstore)
+			hipLaunchKernelGGL(k_stream_groups<true>, dim3((unsigned) (nslots / 1024)),
+					dim3(PROBE_BLOCK), 0, s, a, b, nslots);
+		else
+			hipLaunchKernelGGL(k_stream_groups<false>, dim3((unsigned) (nslots / 1024)),
+					dim3(PROBE_BLOCK), 0, s, a, b, nslots);
 	}
 	if(hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
 			hipEventSynchronize(e1) != hipSuccess ||
