@@ -125,3 +125,46 @@ def test_noalloc_rejects_grids_beyond_its_workspace():
         assert rel_l2(phi, handle_phi(rho)) <= 1e-12
     finally:
         g.free()
+
+
+def test_noalloc_solve_replayed_from_a_captured_graph():
+    """The allocation-free solve captured into a HIP graph (its callers'
+    case) and replayed four times, each on a new rho written between the
+    replays, matches the handle's solve of that rho every time."""
+    h = hip_runtime()
+    ok(h.hipSetDevice(0))
+    h.hipStreamBeginCapture.argtypes = [C.c_void_p, C.c_int]
+    h.hipStreamEndCapture.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    h.hipGraphInstantiate.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_size_t]
+    h.hipGraphLaunch.argtypes = [C.c_void_p, C.c_void_p]
+    h.hipGraphExecDestroy.argtypes = [C.c_void_p]
+    h.hipGraphDestroy.argtypes = [C.c_void_p]
+    lib = ca.lib()
+    rng = np.random.default_rng(11)
+    nx, ny = 256, 128
+    g = DeviceGrid(h, np.zeros((ny, nx)))
+    stream, graph, exe = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ok(h.hipStreamCreate(C.byref(stream)))
+    try:
+        print("capturing cpic_solve_xy_noalloc 256x128", flush=True)
+        ok(h.hipStreamBeginCapture(stream, 0))          # hipStreamCaptureModeGlobal
+        ret = lib.cpic_solve_xy_noalloc(nx, ny, g.rho, g.phi, stream)
+        ok(h.hipStreamEndCapture(stream, C.byref(graph)))
+        ok(ret)
+        ok(h.hipGraphInstantiate(C.byref(exe), graph, None, None, 0))
+        for rep in range(4):
+            rho = rng.normal(size=(ny, nx))
+            ok(h.hipMemcpy(g.rho, rho.ctypes.data, rho.nbytes, H2D))
+            ok(h.hipGraphLaunch(exe, stream))
+            ok(h.hipStreamSynchronize(stream))
+            phi = g.read("phi")
+            err = rel_l2(phi, handle_phi(rho))
+            assert err <= 1e-12, (rep, err)
+    finally:
+        if exe.value:
+            h.hipGraphExecDestroy(exe)
+        if graph.value:
+            h.hipGraphDestroy(graph)
+        h.hipStreamDestroy(stream)
+        g.free()
