@@ -33,6 +33,7 @@
 
 #define CPIC_BLOCK 256
 #define CPIC_MARGIN 1            // cells of drift covered by the LDS tile regions
+#define CPIC_BUCKET_MAX (1 << 27) // slots of one (tile, species) bucket: 32-bit byte offsets
 #define CPIC_MAX_KERNELS 24
 
 // device error word bits

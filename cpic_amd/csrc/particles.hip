@@ -80,10 +80,20 @@ struct PushArgs {
 // plasma's table entry with the host's expressions, so they are the same
 // doubles the reference computes: dtqm2 = 0.5 * dt' * q / m and the
 // rotation factors t, s = 2t/(1+t^2) (src/mover.c:23-83, 204-227).
+// For FP32 particles the velocity limits umax are taken as the largest
+// floats not above them: for a float u and a double U, |u| > U exactly when
+// |u| > that float, so the check_velocity test stays exact in FP32.
 template <typename T> struct SpeciesK {
 	T h, tz, sz, vq;
 	double mh;
+	float umx, umy;
 };
+
+static __device__ __forceinline__ float f32_at_most(double v)
+{
+	float f = (float) v;
+	return (double) f > v ? nextafterf(f, -INFINITY) : f;
+}
 
 template <typename T>
 static __device__ __forceinline__ SpeciesK<T> species_k(const Geo &g, int s, double dtp)
@@ -97,6 +107,8 @@ static __device__ __forceinline__ SpeciesK<T> species_k(const Geo &g, int s, dou
 	k.sz = (T) (2.0 * tz / sden);
 	k.vq = (T) (-g.q[s] / g.e0);
 	k.mh = 0.5 * g.m[s];
+	k.umx = f32_at_most(g.umax_x);
+	k.umy = f32_at_most(g.umax_y);
 	return k;
 }
 
@@ -156,9 +168,26 @@ static __device__ __forceinline__ const GLB_AS T *bucket_base(const Parts &p, in
 	return gptr<T>(p.r) + 4 * bstart;
 }
 
-static __device__ __forceinline__ int64_t pslot_rel(int j)
+// Byte offset of bucket slot j's x from the bucket base.  It fits 32 bits
+// (bucket capacities stay below CPIC_BUCKET_MAX slots, k_scan_caps), so the
+// particle loads and stores take the scalar-base + 32-bit-vector-offset form.
+template <typename T>
+static __device__ __forceinline__ uint32_t pslot_byte(int j)
 {
-	return ((int64_t) (j >> 6) << 8) + (j & 63);
+	return ((((uint32_t) j >> 6) << 8) | ((uint32_t) j & 63u)) * (uint32_t) sizeof(T);
+}
+
+template <typename T>
+static __device__ __forceinline__ GLB_AS T *at_byte(const GLB_AS T *base, uint32_t off)
+{
+	return (GLB_AS T *) ((const GLB_AS char *) base + off);
+}
+
+// Number of bits of m set below this lane.
+static __device__ __forceinline__ int lanes_below(unsigned long long m)
+{
+	return (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
+			__builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
 }
 
 // A block-uniform value read from memory the kernel also writes: the
@@ -487,7 +516,8 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 	T unx = q.ux, uny = q.uy;
 	boris_bz<T>(h, tz, sz, ex, ey, unx, uny);
 	// check_velocity (src/mover.c:98-137)
-	if(fabs((double) unx) > g.umax_x || fabs((double) uny) > g.umax_y)
+	if(sizeof(T) == 4 ? fabsf((float) unx) > k.umx || fabsf((float) uny) > k.umy
+			: fabs((double) unx) > g.umax_x || fabs((double) uny) > g.umax_y)
 		bad = true;
 	double ux0 = (double) q.ux, uy0 = (double) q.uy;
 	double ux1 = (double) unx, uy1 = (double) uny;
@@ -511,8 +541,7 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 			q.y = (T) fmaf((float) uny, dtdy, (float) q.y);
 		}
 	}
-	if(!(isfinite((double) q.x) && isfinite((double) q.y) &&
-			isfinite((double) q.ux) && isfinite((double) q.uy)))
+	if(!(isfinite(q.x) && isfinite(q.y) && isfinite(q.ux) && isfinite(q.uy)))
 	{
 		nan_ = true;
 		q.x = q.y = (T) 0;
@@ -552,7 +581,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 		// was 4 % slower than with the per-slot loads below.
 		typedef typename NU<T, U>::t VU;
 		const int j0 = base + U * tid;
-		const GLB_AS VU *r = (const GLB_AS VU *) (pb + pslot_rel(j0));
+		const GLB_AS VU *r = (const GLB_AS VU *) at_byte(pb, pslot_byte<T>(j0));
 		VU vx = r[0], vy = r[PGROUP / U], vux = r[2 * PGROUP / U], vuy = r[3 * PGROUP / U];
 #pragma unroll
 		for(int u = 0; u < U; u++)
@@ -576,7 +605,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 		if(act[u])
 		{
 			// global-address-space loads (see load_p)
-			const GLB_AS T *r = pb + pslot_rel(jj);
+			const GLB_AS T *r = at_byte(pb, pslot_byte<T>(jj));
 			q[u].x = r[0];
 			q[u].y = r[PGROUP];
 			q[u].ux = r[2 * PGROUP];
@@ -645,7 +674,7 @@ static __device__ __forceinline__ void back_to_tile(const Geo &g, int ndst, int 
 
 // EPHI: the tile's E comes from the potential (the E array is stale).
 template <typename T, bool IDS, bool PE, int U, bool EPHI>
-__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 ? PUSH_MIN_WAVES_F32 : 1)
+__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 && !IDS && !PE ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
@@ -669,7 +698,6 @@ void k_push(PushArgs a)
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
 	int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-	unsigned long long lt = (1ull << lane) - 1ull;
 	const T2 *__restrict__ E = (const T2 *) a.E + g.node0;
 	const double *__restrict__ phi = EPHI ? a.phi + g.node0 : (const double *) NULL;
 	double *__restrict__ rho = a.rho + g.node0;
@@ -735,7 +763,7 @@ void k_push(PushArgs a)
 					{
 						if(leave[u])
 						{
-							int k = sb + __popcll(lm & lt);
+							int k = sb + lanes_below(lm);
 							s_lv.x[k] = q[u].x;
 							s_lv.y[k] = q[u].y;
 							s_lv.ux[k] = q[u].ux;
@@ -762,7 +790,7 @@ void k_push(PushArgs a)
 						lbase = __builtin_amdgcn_readfirstlane(lbase);
 						if(leave[u])
 						{
-							int64_t ls = (int64_t) lbase + __popcll(lm & lt);
+							int64_t ls = (int64_t) lbase + lanes_below(lm);
 							if(ls < a.lv.cap)
 								store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
 							else
@@ -806,8 +834,8 @@ void k_push(PushArgs a)
 				{
 					if(act[u] && !leave[u])
 					{
-						int jj = wb + __popcll(smk & lt);
-						GLB_AS T *r = (GLB_AS T *) bucket_base<T>(a.q, bstart) + pslot_rel(jj);
+						int jj = wb + lanes_below(smk);
+						GLB_AS T *r = at_byte(bucket_base<T>(a.q, bstart), pslot_byte<T>(jj));
 						r[0] = q[u].x;
 						r[PGROUP] = q[u].y;
 						r[2 * PGROUP] = q[u].ux;
@@ -1142,15 +1170,21 @@ void k_hist_stage(const Geo *__restrict__ geo, Staged u, int *__restrict__ cnt)
 	}
 }
 
-// Capacities and exclusive scan of bucket starts (one workgroup).
+// Capacities and exclusive scan of bucket starts (one workgroup).  With cap
+// given, a bucket that would need more than CPIC_BUCKET_MAX slots makes the
+// total -1.
 __global__ __launch_bounds__(1024)
 void k_scan_caps(const int *__restrict__ cnt, int nb, double slack, int pad, int align,
 		int64_t *__restrict__ start, int *__restrict__ cap, int64_t *__restrict__ total)
 {
 	__shared__ int64_t s[1024];
 	__shared__ int64_t carry;
+	__shared__ int big;
 	if(threadIdx.x == 0)
+	{
 		carry = 0;
+		big = 0;
+	}
 	__syncthreads();
 	for(int base = 0; base < nb; base += 1024)
 	{
@@ -1162,7 +1196,11 @@ void k_scan_caps(const int *__restrict__ cnt, int nb, double slack, int pad, int
 			c = n + (int64_t) ceil((double) n * slack) + pad;
 			c = (c + align - 1) / align * align;
 			if(cap)
-				cap[b] = (int) c;
+			{
+				if(c > CPIC_BUCKET_MAX)
+					big = 1;
+				cap[b] = (int) (c > CPIC_BUCKET_MAX ? CPIC_BUCKET_MAX : c);
+			}
 		}
 		s[threadIdx.x] = c;
 		__syncthreads();
@@ -1181,7 +1219,7 @@ void k_scan_caps(const int *__restrict__ cnt, int nb, double slack, int pad, int
 		__syncthreads();
 	}
 	if(threadIdx.x == 0)
-		*total = carry;
+		*total = big ? -1 : carry;
 }
 
 template <typename T, bool IDS, bool PE>
@@ -1729,6 +1767,13 @@ cpic_parts_rebuild(cpic_dev *d)
 		RB_CHK(hipGetLastError());
 		RB_CHK(hipMemcpyAsync(&tot, total, sizeof(int64_t), hipMemcpyDeviceToHost, d->stream));
 		RB_CHK(hipStreamSynchronize(d->stream));
+		if(tot < 0)
+		{
+			fprintf(stderr, "cpic_hip: a (tile, species) bucket would need more than %d slots\n",
+					CPIC_BUCKET_MAX);
+			ret = CPIC_ENOMEM;
+			goto out;
+		}
 
 		int spare = 1 - d->pcur;
 		touched = 1;
