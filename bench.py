@@ -259,7 +259,7 @@ def acc_report(acc):
             "energy_growth": float(o[n - 1] / o[0])}
 
 
-def stream_ceiling(total_bytes, achieved):
+def stream_ceiling(total_bytes, achieved, tile_slots=None):
     """Best rate of a hand-written read+write stream (lib/libcpic_probe.so:
     16 B per element, dwordx4 loads and stores, 1-8 loads in flight per
     thread) moving the push's algorithmic bytes, half read and half written."""
@@ -296,7 +296,23 @@ def stream_ceiling(total_bytes, achieved):
                                    "thread and component, one dword store per slot and component, "
                                    "no arithmetic"),
                    frac_of_pattern_ceiling=achieved / pat if achieved else None)
-    log(f"stream ceiling: {rates}, push pattern: {out.get('pattern_ceiling_GBs')}")
+    # the same pattern streamed tile by tile, one workgroup per tile as the
+    # push runs (tile_slots particles per tile, chunks of 1024)
+    if tile_slots:
+        probe.cpic_probe_stream_tiles.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_double)]
+        ts = max(1024, int(round(tile_slots / 1024)) * 1024)
+        ms = ctypes.c_double()
+        if probe.cpic_probe_stream_tiles(half, ts, 10, ctypes.byref(ms)) == 0 and ms.value > 0:
+            moved = 2 * (half // 16 // ts * ts * 16)
+            tpat = moved / (ms.value / 1e3) / 1e9
+            out.update(tile_pattern_ceiling_GBs=tpat,
+                       tile_pattern_kernel=(f"the same pattern, one 256-thread workgroup per tile of "
+                                            f"{ts} slots streamed in chunks of 1024, tiles in XCD "
+                                            f"bands as the push orders them"),
+                       frac_of_tile_pattern_ceiling=achieved / tpat if achieved else None)
+    log(f"stream ceiling: {rates}, push pattern: {out.get('pattern_ceiling_GBs')}, "
+        f"per tile: {out.get('tile_pattern_ceiling_GBs')}")
     return out
 
 
@@ -646,7 +662,10 @@ def main():
                               for k in common},
             }
     if not args.no_ceiling:
-        out["roofline"].update(stream_ceiling(alg_bytes, achieved))
+        tile = r32["stats"]["tile"]
+        ntiles = (args.grid // tile[0]) * (args.grid // tile[1]) if tile else 0
+        out["roofline"].update(stream_ceiling(alg_bytes, achieved,
+                                              tile_slots=ntot / ntiles if ntiles else None))
     acc = None
     if args.oracle_grid > 0 and args.accuracy_steps > 0:
         acc = fp32_oracle_leg_gpu(ca, args.oracle_grid, args.accuracy_steps, args.seed)

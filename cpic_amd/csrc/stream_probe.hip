@@ -83,6 +83,86 @@ void k_stream_groups(const float *__restrict__ a, float *__restrict__ b, int64_t
 	}
 }
 
+// The same pattern walked the way k_push walks its tiles: each workgroup
+// streams ts slots (one tile's particles) in chunks of 1024, and workgroup b
+// takes tile (b & 7) * (ntiles / 8) + (b >> 3), contiguous bands of tiles
+// per XCD as k_push's tile order does.  Streaming a tile in 8 chunks per
+// workgroup instead of one chunk per workgroup costs about 8 % by itself.
+__global__ __launch_bounds__(PROBE_BLOCK)
+void k_stream_tiles(const float *__restrict__ a, float *__restrict__ b, int ts, int ntiles)
+{
+	int tile = blockIdx.x;
+	if((ntiles & 7) == 0)
+		tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+	const int t = threadIdx.x;
+	const int64_t t0 = (int64_t) tile * ts;
+	for(int c = 0; c < ts; c += 1024)
+	{
+		int64_t base = t0 + c, j0 = base + 4 * t;
+		const float4 *r = (const float4 *) (a + (j0 >> 6) * 256 + (j0 & 63));
+		float4 x = r[0], y = r[16], ux = r[32], uy = r[48];
+		float vx[4] = {x.x, x.y, x.z, x.w}, vy[4] = {y.x, y.y, y.z, y.w};
+		float vux[4] = {ux.x, ux.y, ux.z, ux.w}, vuy[4] = {uy.x, uy.y, uy.z, uy.w};
+#pragma unroll
+		for(int u = 0; u < 4; u++)
+		{
+			int64_t sl = base + u * PROBE_BLOCK + t;
+			float *w = b + (sl >> 6) * 256 + (sl & 63);
+			w[0] = vx[u];
+			w[64] = vy[u];
+			w[128] = vux[u];
+			w[192] = vuy[u];
+		}
+	}
+}
+
+// Average milliseconds per launch of k_stream_tiles over two buffers of
+// `bytes` bytes each, whole tiles of tile_slots slots (a multiple of 1024).
+extern "C" int
+cpic_probe_stream_tiles(int64_t bytes, int tile_slots, int reps, double *ms_avg)
+{
+	if(tile_slots < 1024 || tile_slots % 1024 || reps < 1 || !ms_avg)
+		return -1;
+	int64_t ntiles = bytes / 16 / tile_slots;
+	if(ntiles < 1 || ntiles > (1 << 30))
+		return -1;
+	int64_t nslots = ntiles * tile_slots;
+	float *a = NULL, *b = NULL;
+	hipStream_t s = NULL;
+	hipEvent_t e0 = NULL, e1 = NULL;
+	float ms = 0.0f;
+	int ret = -1;
+	if(hipMalloc(&a, 16 * (size_t) nslots) != hipSuccess ||
+			hipMalloc(&b, 16 * (size_t) nslots) != hipSuccess ||
+			hipMemset(a, 1, 16 * (size_t) nslots) != hipSuccess ||
+			hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+			hipEventCreate(&e1) != hipSuccess)
+		goto out;
+	for(int r = 0; r <= reps; r++)
+	{
+		if(r == 1 && hipEventRecord(e0, s) != hipSuccess)
+			goto out;
+		hipLaunchKernelGGL(k_stream_tiles, dim3((unsigned) ntiles), dim3(PROBE_BLOCK), 0, s, a, b,
+				tile_slots, (int) ntiles);
+	}
+	if(hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+			hipEventSynchronize(e1) != hipSuccess ||
+			hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+		goto out;
+	*ms_avg = (double) ms / reps;
+	ret = 0;
+out:
+	if(ret)
+		fprintf(stderr, "cpic_probe_stream_tiles: HIP error: %s\n",
+				hipGetErrorString(hipGetLastError()));
+	if(e0) hipEventDestroy(e0);
+	if(e1) hipEventDestroy(e1);
+	if(s) hipStreamDestroy(s);
+	hipFree(a);
+	hipFree(b);
+	return ret;
+}
+
 // Average milliseconds per launch of k_stream_groups over two buffers of
 // `bytes` bytes each (whole chunks of 1024 slots; 16 B per slot), as for
 // cpic_probe_stream; vstore = 1 writes with dwordx4 stores instead of the

@@ -26,6 +26,20 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_probe_library_exports_the_bench_ceilings():
+    """bench.py's streaming ceilings come from lib/libcpic_probe.so (not part
+    of the ABI): plain stream, the push's pattern, the same per tile."""
+    import ctypes
+    probe = ctypes.CDLL(os.path.join(ROOT, "cpic_amd", "lib", "libcpic_probe.so"))
+    for s in ("cpic_probe_stream", "cpic_probe_stream_groups", "cpic_probe_stream_tiles"):
+        assert hasattr(probe, s), s
+    ms = ctypes.c_double()
+    # argument checks run before any device call
+    assert probe.cpic_probe_stream_tiles(ctypes.c_int64(1 << 20), 1000, 1, ctypes.byref(ms)) == -1
+    assert probe.cpic_probe_stream_tiles(ctypes.c_int64(1 << 20), 1024, 0, ctypes.byref(ms)) == -1
+    assert probe.cpic_probe_stream(ctypes.c_int64(1 << 20), 3, 1, ctypes.byref(ms)) == -1
+
+
 def test_strerror_and_rho_size():
     lib = ca.lib()
     for code in ca.ERRORS:
