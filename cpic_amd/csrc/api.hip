@@ -163,6 +163,8 @@ dev_free(cpic_dev *d)
 	hipFree(d->bk.start); hipFree(d->bk.cap); hipFree(d->bk.n1); hipFree(d->bk.res);
 	hipFree(d->bk.n2);
 	hipFree(d->order);
+	hipFree(d->border);
+	hipFree(d->dpart);
 	hipFree(d->lv.x); hipFree(d->lv.y); hipFree(d->lv.ux); hipFree(d->lv.uy);
 	hipFree(d->lv.id); hipFree(d->lv.Ex); hipFree(d->lv.Ey); hipFree(d->lv.dst);
 	hipFree(d->lv.src); hipFree(d->lv.count);
@@ -447,6 +449,8 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 			(ret = alloc_zero((void **) &d->geo, sizeof(Geo) * P)) ||
 			(ret = alloc_zero((void **) &d->pre, sizeof(int) * PRE_COUNT * (P + 1))) ||
 			(ret = alloc_zero((void **) &d->order, sizeof(int) * d->ntiles)) ||
+			(ret = alloc_zero((void **) &d->border, sizeof(int) * d->nb)) ||
+			(ret = alloc_zero((void **) &d->dpart, sizeof(double) * DIAG_SPREAD * DIAG_LINE * P)) ||
 			(ret = alloc_zero((void **) &d->bk.start, sizeof(int64_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.cap, sizeof(int32_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.n1, sizeof(int32_t) * d->nb)) ||

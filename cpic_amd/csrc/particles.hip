@@ -1,12 +1,12 @@
 // Particle kernels of the gfx950 PIC step.
 //
-// k_push: one workgroup per tile.  Stages the tile's E region (tile + 1 cell
-// margin + the CIC +1 node) in LDS -- after a field solve it computes that
-// region from the potential (E = -grad phi, src/field.c:381-406, staged with a
-// one-node halo), so a step never writes or reads a global E array -- streams
-// every particle of the tile's
-// buckets once (coalesced loads of the state, stored in groups of 64:
-// x[64] y[64] ux[64] uy[64]), and for each particle
+// k_push: one workgroup per (tile, species) bucket.  Stages the tile's E
+// region (tile + 1 cell margin + the CIC +1 node) in LDS -- after a field
+// solve it computes that region from the potential (E = -grad phi,
+// src/field.c:381-406, staged with a one-node halo), so a step never writes
+// or reads a global E array -- streams every particle of the bucket once
+// (coalesced loads of the state, stored in groups of 64: x[64] y[64] ux[64]
+// uy[64]), and for each particle
 //   gathers E with CIC weights    (interpolate_f2p_E, src/interpolate.c:350-404)
 //   applies the Boris push        (boris_rotation/move, src/mover.c:23-83, 141-227)
 //   wraps periodically            (periodic_boundary_ppack, src/comm_plasma.c:726-747)
@@ -23,7 +23,7 @@
 //   leavers (particles whose cell moved to another tile: comm_plasma's
 //   collect, src/comm_plasma.c:101-165) wait in an LDS stage and are copied
 //   to a leaver buffer with one global atomic per bucket.
-// The LDS rho region is flushed once per tile with FP64 global atomics.
+// The LDS rho region is flushed once per workgroup with FP64 global atomics.
 // k_append then places each leaver at the end of its destination bucket
 // (inject_particles, src/comm_plasma.c:358-400) or, when that bucket is full,
 // back into a reserved slot of its source bucket, so no particle is ever lost
@@ -40,6 +40,7 @@
 #include "picmath.h"
 
 #include <algorithm>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -57,7 +58,9 @@ template <typename T, int U> struct NU { typedef T t __attribute__((ext_vector_t
 struct PushArgs {
 	const Geo *geo;
 	const int *pre_tile;  // [P+1] first global tile of each plasma
-	const int *order;     // [ntiles] tile of each block (cpic_parts_order)
+	const int *order;     // [ntiles] tile of each block (cpic_parts_order); k_push:
+	                      // [nb] bucket of each block
+	const int *pre_bucket; // [P+1] first global bucket of each plasma
 	int P;
 	int ntiles;           // tiles of the whole batch
 	Buckets bk;
@@ -70,6 +73,7 @@ struct PushArgs {
 	const double *phi;    // potential, whole batch (pushes that compute E)
 	double *rho;          // deposit target, whole batch
 	DiagSlot *diag;       // nullable: per-plasma rings of `ring` slots
+	double *dpart;        // with diag: [P][DIAG_SPREAD][DIAG_LINE] accumulators
 	int ring;
 	const StepState *st;
 	int *derr;
@@ -181,6 +185,33 @@ template <typename T>
 static __device__ __forceinline__ GLB_AS T *at_byte(const GLB_AS T *base, uint32_t off)
 {
 	return (GLB_AS T *) ((const GLB_AS char *) base + off);
+}
+
+// One DPP row operation (CTRL, rows ROW_MASK) on both halves of a double;
+// lanes without a source get 0.
+template <int CTRL, int ROW_MASK>
+static __device__ __forceinline__ double dpp_f64(double v)
+{
+	int lo = __double2loint(v), hi = __double2hiint(v);
+	lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+	hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+	return __hiloint2double(hi, lo);
+}
+
+// Sum of v over a full wavefront, left in lane 63: an inclusive scan with
+// row_shr 1, 2, 4, 8 inside each row of 16 lanes, then row_bcast 15 (rows
+// 1 and 3 add the last lane of the row before) and row_bcast 31 (rows 2
+// and 3 add lane 31).  No LDS traffic, unlike shuffles through
+// ds_bpermute.
+static __device__ __forceinline__ double wave_sum_lane63(double v)
+{
+	v += dpp_f64<0x111, 0xf>(v);
+	v += dpp_f64<0x112, 0xf>(v);
+	v += dpp_f64<0x114, 0xf>(v);
+	v += dpp_f64<0x118, 0xf>(v);
+	v += dpp_f64<0x142, 0xa>(v);
+	v += dpp_f64<0x143, 0xc>(v);
+	return v;
 }
 
 // Number of bits of m set below this lane.
@@ -681,11 +712,14 @@ void k_push(PushArgs a)
 	extern __shared__ unsigned char smem[];
 	__shared__ int s_kcount;
 	__shared__ int s_w;
-	__shared__ double s_red[3][CPIC_BLOCK / 64];
 	__shared__ LvStage<T, IDS, PE> s_lv;
 
-	int gt = a.order[blockIdx.x];
-	int pl = find_plasma(a.pre_tile, a.P, gt);
+	// one (tile, species) bucket per block: the tile's species run as
+	// separate blocks (fewer chunks per block than one block per tile: a
+	// stream of this access pattern measured 1.60 ms at 8 chunks per block
+	// and 1.55 ms at 4; the push 3 % faster in FP32, 3.5 % in FP64)
+	const int unit = a.order[blockIdx.x];
+	const int pl = find_plasma(a.pre_bucket, a.P, unit);
 	// the plasma's table entry, read once into registers (a reference into
 	// global memory would be re-read after every particle store); the
 	// per-species arrays are read through gp
@@ -694,7 +728,9 @@ void k_push(PushArgs a)
 	T2 *sE = (T2 *) smem;
 	double *sR = (double *) (smem + sizeof(T2) * g.W * g.H);
 
-	int tile = gt - g.tile0;
+	const int b = unit;
+	const int tile = (b - g.bucket0) / g.ns;
+	const int s = b - g.bucket0 - tile * g.ns;
 	int tix = tile & (g.ntx - 1), tiy = tile >> g.lgntx;
 	int ox = tix * g.tx - g.M, oy = tiy * g.ty - g.M;
 	int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -704,6 +740,8 @@ void k_push(PushArgs a)
 	const double dtp = a.first ? -g.dt / 2 : g.dt;
 	const bool set_r = !a.first;
 	const float dtdx = (float) (dtp / g.dx), dtdy = (float) (dtp / g.dy);
+	// read here, used at the end: its load latency then overlaps the work
+	const long long iter = a.diag ? a.st->iter : -1;
 
 	if(EPHI)
 		tile_setup_phi<T>(g, ox, oy, phi, sE, sR);
@@ -721,189 +759,180 @@ void k_push(PushArgs a)
 	double ke = 0.0, mpx = 0.0, mpy = 0.0;
 	bool bad = false, nan_ = false;
 
-	for(int s = 0; s < g.ns; s++)
-	{
-		const int b = g.bucket0 + tile * g.ns + s;
-		const int64_t bstart = uniform(a.bk.start[b]);
-		const int cap = uniform(a.bk.cap[b]), res = uniform(a.bk.res[b]);
-		const int n1 = min(uniform(a.bk.n1[b]), cap - res);
-		const int ntot = n1 + uniform(a.bk.n2[b]);
-		const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
-		const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
+	const int64_t bstart = uniform(a.bk.start[b]);
+	const int cap = uniform(a.bk.cap[b]), res = uniform(a.bk.res[b]);
+	const int n1 = min(uniform(a.bk.n1[b]), cap - res);
+	const int ntot = n1 + uniform(a.bk.n2[b]);
+	const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
+	const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 
-		for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
+	for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
+	{
+		PState<T> q[U];
+		bool act[U], leave[U];
+		int ndst[U];
+		load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
+#pragma unroll
+		for(int u = 0; u < U; u++)
 		{
-			PState<T> q[U];
-			bool act[U], leave[U];
-			int ndst[U];
-			load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
+			leave[u] = false;
+			ndst[u] = 0;
+			if(act[u])
+				leave[u] = push_one<T, EPHI>(g, sE, E, phi, sR, rho, tile, ox, oy, s, sk, set_r,
+						dtp, dtdx, dtdy, q[u], ndst[u], ke, mpx, mpy, bad, nan_);
+		}
+		// leavers: staged in LDS (one LDS atomic per wavefront); once
+		// the stage is full, placed directly (one global atomic each)
 #pragma unroll
-			for(int u = 0; u < U; u++)
+		for(int u = 0; u < U; u++)
+		{
+			unsigned long long lm = __ballot(leave[u]);
+			if(lm)
 			{
-				leave[u] = false;
-				ndst[u] = 0;
-				if(act[u])
-					leave[u] = push_one<T, EPHI>(g, sE, E, phi, sR, rho, tile, ox, oy, s, sk, set_r,
-							dtp, dtdx, dtdy, q[u], ndst[u], ke, mpx, mpy, bad, nan_);
-			}
-			// leavers: staged in LDS (one LDS atomic per wavefront); once
-			// the stage is full, placed directly (one global atomic each)
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				unsigned long long lm = __ballot(leave[u]);
-				if(lm)
-				{
-					int lcnt = __popcll(lm);
-					int sb = 0;
-					if(lane == 0)
-						sb = atomicAdd(&s_lv.n, lcnt);
-					sb = __builtin_amdgcn_readfirstlane(sb);
-					if(sb + lcnt <= LV_STAGE)
-					{
-						if(leave[u])
-						{
-							int k = sb + lanes_below(lm);
-							s_lv.x[k] = q[u].x;
-							s_lv.y[k] = q[u].y;
-							s_lv.ux[k] = q[u].ux;
-							s_lv.uy[k] = q[u].uy;
-							s_lv.dst[k] = ndst[u];
-							if(IDS)
-								s_lv.id[k] = q[u].id;
-							if(PE)
-							{
-								s_lv.Ex[k] = q[u].Ex;
-								s_lv.Ey[k] = q[u].Ey;
-							}
-						}
-					}
-					else
-					{
-						// records [sb, LV_STAGE) stay unused; later
-						// wavefronts all start at or beyond LV_STAGE
-						if(lane == 0 && sb < LV_STAGE)
-							s_lv.cut = sb;
-						int lbase = 0;
-						if(lane == 0)
-							lbase = atomicAdd(&a.lv.count[a.par], lcnt);
-						lbase = __builtin_amdgcn_readfirstlane(lbase);
-						if(leave[u])
-						{
-							int64_t ls = (int64_t) lbase + lanes_below(lm);
-							if(ls < a.lv.cap)
-								store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
-							else
-							{
-								// no room: stays in this bucket (misplaced)
-								leave[u] = false;
-								back_to_tile<T>(g, ndst[u], ox, oy, q[u]);
-								atomicAdd(&a.ctr->misplaced, 1ull);
-							}
-						}
-						unsigned long long placed = __ballot(leave[u]);
-						if(lane == 0 && placed)
-							atomicAdd(&s_kcount, __popcll(placed));
-					}
-				}
-			}
-			// stayers: one LDS atomic per wavefront and chunk claims their
-			// slots in the other buffer for all U rounds (order within the
-			// bucket is not kept)
-			unsigned long long smks[U];
-			int wbase = 0, wtot = 0;
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				smks[u] = __ballot(act[u] && !leave[u]);
-				wtot += __popcll(smks[u]);
-			}
-			if(wtot)
-			{
+				int lcnt = __popcll(lm);
+				int sb = 0;
 				if(lane == 0)
-					wbase = atomicAdd(&s_w, wtot);
-				wbase = __builtin_amdgcn_readfirstlane(wbase);
-			}
-#pragma unroll
-			for(int u = 0; u < U; u++)
-			{
-				unsigned long long smk = smks[u];
-				int wb = wbase;
-				wbase += __popcll(smk);
-				if(smk)
+					sb = atomicAdd(&s_lv.n, lcnt);
+				sb = __builtin_amdgcn_readfirstlane(sb);
+				if(sb + lcnt <= LV_STAGE)
 				{
-					if(act[u] && !leave[u])
+					if(leave[u])
 					{
-						int jj = wb + lanes_below(smk);
-						GLB_AS T *r = at_byte(bucket_base<T>(a.q, bstart), pslot_byte<T>(jj));
-						r[0] = q[u].x;
-						r[PGROUP] = q[u].y;
-						r[2 * PGROUP] = q[u].ux;
-						r[3 * PGROUP] = q[u].uy;
+						int k = sb + lanes_below(lm);
+						s_lv.x[k] = q[u].x;
+						s_lv.y[k] = q[u].y;
+						s_lv.ux[k] = q[u].ux;
+						s_lv.uy[k] = q[u].uy;
+						s_lv.dst[k] = ndst[u];
 						if(IDS)
-							a.q.id[bstart + jj] = q[u].id;
+							s_lv.id[k] = q[u].id;
 						if(PE)
 						{
-							a.q.Ex[bstart + jj] = q[u].Ex;
-							a.q.Ey[bstart + jj] = q[u].Ey;
+							s_lv.Ex[k] = q[u].Ex;
+							s_lv.Ey[k] = q[u].Ey;
 						}
+					}
+				}
+				else
+				{
+					// records [sb, LV_STAGE) stay unused; later
+					// wavefronts all start at or beyond LV_STAGE
+					if(lane == 0 && sb < LV_STAGE)
+						s_lv.cut = sb;
+					int lbase = 0;
+					if(lane == 0)
+						lbase = atomicAdd(&a.lv.count[a.par], lcnt);
+					lbase = __builtin_amdgcn_readfirstlane(lbase);
+					if(leave[u])
+					{
+						int64_t ls = (int64_t) lbase + lanes_below(lm);
+						if(ls < a.lv.cap)
+							store_l<T>(a.lv, ls, q[u], IDS, PE, ndst[u], b);
+						else
+						{
+							// no room: stays in this bucket (misplaced)
+							leave[u] = false;
+							back_to_tile<T>(g, ndst[u], ox, oy, q[u]);
+							atomicAdd(&a.ctr->misplaced, 1ull);
+						}
+					}
+					unsigned long long placed = __ballot(leave[u]);
+					if(lane == 0 && placed)
+						atomicAdd(&s_kcount, __popcll(placed));
+				}
+			}
+		}
+		// stayers: one LDS atomic per wavefront and chunk claims their
+		// slots in the other buffer for all U rounds (order within the
+		// bucket is not kept)
+		unsigned long long smks[U];
+		int wbase = 0, wtot = 0;
+#pragma unroll
+		for(int u = 0; u < U; u++)
+		{
+			smks[u] = __ballot(act[u] && !leave[u]);
+			wtot += __popcll(smks[u]);
+		}
+		if(wtot)
+		{
+			if(lane == 0)
+				wbase = atomicAdd(&s_w, wtot);
+			wbase = __builtin_amdgcn_readfirstlane(wbase);
+		}
+#pragma unroll
+		for(int u = 0; u < U; u++)
+		{
+			unsigned long long smk = smks[u];
+			int wb = wbase;
+			wbase += __popcll(smk);
+			if(smk)
+			{
+				if(act[u] && !leave[u])
+				{
+					int jj = wb + lanes_below(smk);
+					GLB_AS T *r = at_byte(bucket_base<T>(a.q, bstart), pslot_byte<T>(jj));
+					r[0] = q[u].x;
+					r[PGROUP] = q[u].y;
+					r[2 * PGROUP] = q[u].ux;
+					r[3 * PGROUP] = q[u].uy;
+					if(IDS)
+						a.q.id[bstart + jj] = q[u].id;
+					if(PE)
+					{
+						a.q.Ex[bstart + jj] = q[u].Ex;
+						a.q.Ey[bstart + jj] = q[u].Ey;
 					}
 				}
 			}
 		}
+	}
 
-		// the bucket's staged leavers: one global atomic claims their slots
-		__syncthreads();
-		if(tid == 0)
+	// the bucket's staged leavers: one global atomic claims their slots
+	__syncthreads();
+	if(tid == 0)
+	{
+		int nst = min(s_lv.n, s_lv.cut);
+		s_lv.n = nst;
+		s_lv.base = nst > 0 ? atomicAdd(&a.lv.count[a.par], nst) : 0;
+		if(nst > 0)
 		{
-			int nst = min(s_lv.n, s_lv.cut);
-			s_lv.n = nst;
-			s_lv.base = nst > 0 ? atomicAdd(&a.lv.count[a.par], nst) : 0;
-			if(nst > 0)
+			int64_t room = a.lv.cap - (int64_t) s_lv.base;
+			s_kcount += (int) (room <= 0 ? 0 : room < nst ? room : nst);
+		}
+	}
+	__syncthreads();
+	{
+		const int nst = s_lv.n;
+		const int64_t lb0 = s_lv.base;
+		for(int k = tid; k < nst; k += CPIC_BLOCK)
+		{
+			PState<T> lq;
+			lq.x = s_lv.x[k];
+			lq.y = s_lv.y[k];
+			lq.ux = s_lv.ux[k];
+			lq.uy = s_lv.uy[k];
+			lq.id = IDS ? s_lv.id[k] : 0;
+			lq.Ex = PE ? s_lv.Ex[k] : 0.0;
+			lq.Ey = PE ? s_lv.Ey[k] : 0.0;
+			int nd = s_lv.dst[k];
+			if(lb0 + k < a.lv.cap)
+				store_l<T>(a.lv, lb0 + k, lq, IDS, PE, nd, b);
+			else
 			{
-				int64_t room = a.lv.cap - (int64_t) s_lv.base;
-				s_kcount += (int) (room <= 0 ? 0 : room < nst ? room : nst);
+				// no room: stays in this bucket (misplaced)
+				back_to_tile<T>(g, nd, ox, oy, lq);
+				int w = atomicAdd(&s_w, 1);
+				store_p<T>(a.q, bstart + w, lq, IDS, PE);
+				atomicAdd(&a.ctr->misplaced, 1ull);
 			}
 		}
-		__syncthreads();
-		{
-			const int nst = s_lv.n;
-			const int64_t lb0 = s_lv.base;
-			for(int k = tid; k < nst; k += CPIC_BLOCK)
-			{
-				PState<T> lq;
-				lq.x = s_lv.x[k];
-				lq.y = s_lv.y[k];
-				lq.ux = s_lv.ux[k];
-				lq.uy = s_lv.uy[k];
-				lq.id = IDS ? s_lv.id[k] : 0;
-				lq.Ex = PE ? s_lv.Ex[k] : 0.0;
-				lq.Ey = PE ? s_lv.Ey[k] : 0.0;
-				int nd = s_lv.dst[k];
-				if(lb0 + k < a.lv.cap)
-					store_l<T>(a.lv, lb0 + k, lq, IDS, PE, nd, b);
-				else
-				{
-					// no room: stays in this bucket (misplaced)
-					back_to_tile<T>(g, nd, ox, oy, lq);
-					int w = atomicAdd(&s_w, 1);
-					store_p<T>(a.q, bstart + w, lq, IDS, PE);
-					atomicAdd(&a.ctr->misplaced, 1ull);
-				}
-			}
-		}
-		__syncthreads();
-		if(tid == 0)
-		{
-			a.bk.n1[b] = s_w;
-			s_w = 0;
-			a.bk.res[b] = s_kcount;
-			a.bk.n2[b] = 0;
-			s_kcount = 0;
-			s_lv.n = 0;
-			s_lv.cut = LV_STAGE;
-		}
-		__syncthreads();
+	}
+	__syncthreads();
+	if(tid == 0)
+	{
+		a.bk.n1[b] = s_w;
+		a.bk.res[b] = s_kcount;
+		a.bk.n2[b] = 0;
 	}
 
 	if(bad)
@@ -913,36 +942,52 @@ void k_push(PushArgs a)
 
 	tile_flush<T>(g, ox, oy, sR, rho);
 
-	long long iter = a.diag ? a.st->iter : -1;
 	if(iter >= 0)
 	{
-		for(int off = 32; off > 0; off >>= 1)
+		// wavefront sums, then one of the plasma's DIAG_SPREAD accumulators
+		// per wavefront (k_diag_fold adds them into the plasma's slot):
+		// atomics from every block onto the slot itself serialise, 1 % of
+		// the push with a block per tile and 60 % with a block per bucket
+		ke = wave_sum_lane63(ke);
+		mpx = wave_sum_lane63(mpx);
+		mpy = wave_sum_lane63(mpy);
+		if(lane == 63)
 		{
-			ke += __shfl_down(ke, off);
-			mpx += __shfl_down(mpx, off);
-			mpy += __shfl_down(mpy, off);
+			int slot = (blockIdx.x * (CPIC_BLOCK / 64) + wid) & (DIAG_SPREAD - 1);
+			double *acc = a.dpart + ((int64_t) pl * DIAG_SPREAD + slot) * DIAG_LINE;
+			unsafeAtomicAdd(&acc[0], ke);
+			unsafeAtomicAdd(&acc[1], mpx);
+			unsafeAtomicAdd(&acc[2], mpy);
 		}
-		if(lane == 0)
-		{
-			s_red[0][wid] = ke;
-			s_red[1][wid] = mpx;
-			s_red[2][wid] = mpy;
-		}
-		__syncthreads();
-		if(tid == 0)
-		{
-			double k = 0, x = 0, y = 0;
-			for(int i = 0; i < CPIC_BLOCK / 64; i++)
-			{
-				k += s_red[0][i];
-				x += s_red[1][i];
-				y += s_red[2][i];
-			}
-			DiagSlot *ds = &a.diag[g.diag0 + iter % a.ring];
-			unsafeAtomicAdd(&ds->kinetic, k);
-			unsafeAtomicAdd(&ds->px, x);
-			unsafeAtomicAdd(&ds->py, y);
-		}
+	}
+}
+
+// Adds the push's DIAG_SPREAD accumulators of plasma blockIdx.x into its
+// diagnostic slot of the current iteration and clears them for the next.
+__global__ __launch_bounds__(64)
+void k_diag_fold(const Geo *__restrict__ geo, double *__restrict__ dpart,
+		DiagSlot *__restrict__ diag, int ring, const StepState *__restrict__ st)
+{
+	static_assert(DIAG_SPREAD == 64, "one lane per accumulator");
+	long long iter = st->iter;
+	if(iter < 0)
+		return;
+	const int p = blockIdx.x;
+	double *acc = dpart + ((int64_t) p * DIAG_SPREAD + threadIdx.x) * DIAG_LINE;
+	double k = acc[0], x = acc[1], y = acc[2];
+	acc[0] = acc[1] = acc[2] = 0.0;
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		k += __shfl_down(k, off);
+		x += __shfl_down(x, off);
+		y += __shfl_down(y, off);
+	}
+	if(threadIdx.x == 0)
+	{
+		DiagSlot *ds = &diag[geo[p].diag0 + iter % ring];
+		ds->kinetic += k;
+		ds->px += x;
+		ds->py += y;
 	}
 }
 
@@ -1498,6 +1543,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	memset(&a, 0, sizeof(a));
 	a.geo = d->geo;
 	a.pre_tile = PRE(d, PRE_TILE);
+	a.pre_bucket = PRE(d, PRE_BUCKET);
 	a.order = d->order;
 	a.P = d->P;
 	a.ntiles = d->ntiles;
@@ -1511,6 +1557,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	a.phi = d->phi;
 	a.rho = rho;
 	a.diag = (d->flags & CPIC_F_NO_DIAG) ? (DiagSlot *) NULL : d->diag;
+	a.dpart = d->dpart;
 	a.ring = d->ring;
 	a.st = d->st;
 	a.derr = d->derr;
@@ -1553,13 +1600,22 @@ cpic_parts_push(cpic_dev *d, int first_iter)
 	PushArgs a = push_args(d, first_iter, d->rho[1 - d->rho_cur]);
 	int slot;
 
+	a.order = d->border;
 	slot = cpic_kt_begin(d, "push_gather_deposit");
 	if(d->prec == 64)
-		DISPATCH_U(k_push, double, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
+		DISPATCH_U(k_push, double, dim3(d->nb), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	else
-		DISPATCH_U(k_push, float, dim3(d->ntiles), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
+		DISPATCH_U(k_push, float, dim3(d->nb), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
+	if(a.diag)
+	{
+		slot = cpic_kt_begin(d, "diag_fold");
+		hipLaunchKernelGGL(k_diag_fold, dim3(d->P), dim3(64), 0, d->stream, d->geo, d->dpart,
+				a.diag, d->ring, d->st);
+		cpic_kt_end(d, slot);
+		HIPCHK(hipGetLastError());
+	}
 	d->pcur = 1 - d->pcur;
 
 	slot = cpic_kt_begin(d, "append_leavers");
@@ -1639,6 +1695,30 @@ cpic_parts_order(cpic_dev *d, const int *cnt)
 	}
 	hipError_t e = hipMemcpy(d->order, ord, sizeof(int) * (size_t) nt, hipMemcpyHostToDevice);
 	free(ord);
+	if(e != hipSuccess)
+		return CPIC_EHIP;
+	// the same for the buckets (a push by buckets): XCD bands over buckets,
+	// so the buckets of one tile run next to each other on one XCD, largest
+	// first when the counts are uneven
+	const int nb = d->nb;
+	int *bord = (int *) malloc(sizeof(int) * (size_t) nb);
+	if(!bord)
+		return CPIC_ENOMEM;
+	for(int b = 0; b < nb; b++)
+		bord[b] = tile_map(b, nb);
+	if(cnt)
+	{
+		int lo = INT_MAX, hi = 0;
+		for(int b = 0; b < nb; b++)
+		{
+			lo = cnt[b] < lo ? cnt[b] : lo;
+			hi = cnt[b] > hi ? cnt[b] : hi;
+		}
+		if(hi > lo + lo / 2)
+			std::stable_sort(bord, bord + nb, [cnt](int a, int b) { return cnt[a] > cnt[b]; });
+	}
+	e = hipMemcpy(d->border, bord, sizeof(int) * (size_t) nb, hipMemcpyHostToDevice);
+	free(bord);
 	return e == hipSuccess ? 0 : CPIC_EHIP;
 }
 
