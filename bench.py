@@ -296,8 +296,8 @@ def stream_ceiling(total_bytes, achieved, tile_slots=None):
                                    "thread and component, one dword store per slot and component, "
                                    "no arithmetic"),
                    frac_of_pattern_ceiling=achieved / pat if achieved else None)
-    # the same pattern streamed tile by tile, one workgroup per tile as the
-    # push runs (tile_slots particles per tile, chunks of 1024)
+    # the same pattern streamed bucket by bucket, one workgroup per bucket as
+    # the push runs (tile_slots particles per bucket, chunks of 1024)
     if tile_slots:
         probe.cpic_probe_stream_tiles.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_double)]
@@ -307,9 +307,9 @@ def stream_ceiling(total_bytes, achieved, tile_slots=None):
             moved = 2 * (half // 16 // ts * ts * 16)
             tpat = moved / (ms.value / 1e3) / 1e9
             out.update(tile_pattern_ceiling_GBs=tpat,
-                       tile_pattern_kernel=(f"the same pattern, one 256-thread workgroup per tile of "
-                                            f"{ts} slots streamed in chunks of 1024, tiles in XCD "
-                                            f"bands as the push orders them"),
+                       tile_pattern_kernel=(f"the same pattern, one 256-thread workgroup per "
+                                            f"bucket of {ts} slots streamed in chunks of 1024, in "
+                                            f"XCD bands as the push orders them"),
                        frac_of_tile_pattern_ceiling=achieved / tpat if achieved else None)
     log(f"stream ceiling: {rates}, push pattern: {out.get('pattern_ceiling_GBs')}, "
         f"per tile: {out.get('tile_pattern_ceiling_GBs')}")
@@ -662,10 +662,11 @@ def main():
                               for k in common},
             }
     if not args.no_ceiling:
+        # the push runs one workgroup per (tile, species) bucket
         tile = r32["stats"]["tile"]
-        ntiles = (args.grid // tile[0]) * (args.grid // tile[1]) if tile else 0
+        nbuckets = 2 * (args.grid // tile[0]) * (args.grid // tile[1]) if tile else 0
         out["roofline"].update(stream_ceiling(alg_bytes, achieved,
-                                              tile_slots=ntot / ntiles if ntiles else None))
+                                              tile_slots=ntot / nbuckets if nbuckets else None))
     acc = None
     if args.oracle_grid > 0 and args.accuracy_steps > 0:
         acc = fp32_oracle_leg_gpu(ca, args.oracle_grid, args.accuracy_steps, args.seed)
