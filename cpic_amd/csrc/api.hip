@@ -450,7 +450,8 @@ batch_create(const cpic_dev_desc_t *descs, int P, cpic_dev_t **out)
 			(ret = alloc_zero((void **) &d->pre, sizeof(int) * PRE_COUNT * (P + 1))) ||
 			(ret = alloc_zero((void **) &d->order, sizeof(int) * d->ntiles)) ||
 			(ret = alloc_zero((void **) &d->border, sizeof(int) * d->nb)) ||
-			(ret = alloc_zero((void **) &d->dpart, sizeof(double) * DIAG_SPREAD * DIAG_LINE * P)) ||
+			(ret = alloc_zero((void **) &d->dpart,
+					sizeof(double) * (DIAG_SPREAD * DIAG_LINE * (size_t) P + DIAG_LINE))) ||
 			(ret = alloc_zero((void **) &d->bk.start, sizeof(int64_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.cap, sizeof(int32_t) * d->nb)) ||
 			(ret = alloc_zero((void **) &d->bk.n1, sizeof(int32_t) * d->nb)) ||

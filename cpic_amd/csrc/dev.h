@@ -107,9 +107,9 @@ struct Counters {            // device-side statistics, reset by the host
 };
 
 // The push's kinetic and momentum sums go to DIAG_SPREAD accumulators per
-// plasma, each in a 128-byte line of its own (block b adds to accumulator b
-// mod DIAG_SPREAD), which k_diag_fold adds into the plasma's slot: atomics
-// from every block onto one address serialise.
+// plasma, each in a 128-byte line of its own (wavefront w of block b adds to
+// accumulator (4b + w) mod DIAG_SPREAD), which k_append adds into the
+// plasma's slot: atomics from every block onto one address serialise.
 #define DIAG_SPREAD 64
 #define DIAG_LINE 16
 
@@ -177,7 +177,8 @@ struct cpic_dev {
 	Buckets bk;
 	int *order;              // [ntiles] tile taken by each block of the tile kernels
 	int *border;             // [nb] bucket taken by each block of the push
-	double *dpart;           // [P][DIAG_SPREAD][DIAG_LINE]: the push's kinetic, px, py sums
+	double *dpart;           // [P][DIAG_SPREAD][DIAG_LINE]: the push's kinetic, px, py sums,
+	                         // then one line whose first 8 bytes hold their iteration
 	Leavers lv;
 	int step_parity;
 	int64_t *count;          // [P * CPIC_MAX_SPECIES]

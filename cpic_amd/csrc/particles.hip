@@ -74,6 +74,7 @@ struct PushArgs {
 	double *rho;          // deposit target, whole batch
 	DiagSlot *diag;       // nullable: per-plasma rings of `ring` slots
 	double *dpart;        // with diag: [P][DIAG_SPREAD][DIAG_LINE] accumulators
+	long long *diag_iter; // with diag: the iteration the accumulators are for
 	int ring;
 	const StepState *st;
 	int *derr;
@@ -942,10 +943,12 @@ void k_push(PushArgs a)
 
 	tile_flush<T>(g, ox, oy, sR, rho);
 
+	if(a.diag && blockIdx.x == 0 && tid == 0)
+		*a.diag_iter = iter;
 	if(iter >= 0)
 	{
 		// wavefront sums, then one of the plasma's DIAG_SPREAD accumulators
-		// per wavefront (k_diag_fold adds them into the plasma's slot):
+		// per wavefront (k_append adds them into the plasma's slot):
 		// atomics from every block onto the slot itself serialise, 1 % of
 		// the push with a block per tile and 60 % with a block per bucket
 		ke = wave_sum_lane63(ke);
@@ -962,29 +965,22 @@ void k_push(PushArgs a)
 	}
 }
 
-// Adds the push's DIAG_SPREAD accumulators of plasma blockIdx.x into its
-// diagnostic slot of the current iteration and clears them for the next.
-__global__ __launch_bounds__(64)
-void k_diag_fold(const Geo *__restrict__ geo, double *__restrict__ dpart,
-		DiagSlot *__restrict__ diag, int ring, const StepState *__restrict__ st)
+// Adds the push's DIAG_SPREAD accumulators of plasma p into its diagnostic
+// slot of iteration iter and clears them for the next push (one full
+// wavefront, one lane per accumulator).
+static __device__ __forceinline__ void diag_fold(const Geo &g, int p, double *__restrict__ dpart,
+		DiagSlot *__restrict__ diag, int ring, long long iter, int lane)
 {
 	static_assert(DIAG_SPREAD == 64, "one lane per accumulator");
-	long long iter = st->iter;
-	if(iter < 0)
-		return;
-	const int p = blockIdx.x;
-	double *acc = dpart + ((int64_t) p * DIAG_SPREAD + threadIdx.x) * DIAG_LINE;
+	double *acc = dpart + ((int64_t) p * DIAG_SPREAD + lane) * DIAG_LINE;
 	double k = acc[0], x = acc[1], y = acc[2];
 	acc[0] = acc[1] = acc[2] = 0.0;
-	for(int off = 32; off > 0; off >>= 1)
+	k = wave_sum_lane63(k);
+	x = wave_sum_lane63(x);
+	y = wave_sum_lane63(y);
+	if(lane == 63)
 	{
-		k += __shfl_down(k, off);
-		x += __shfl_down(x, off);
-		y += __shfl_down(y, off);
-	}
-	if(threadIdx.x == 0)
-	{
-		DiagSlot *ds = &diag[geo[p].diag0 + iter % ring];
+		DiagSlot *ds = &diag[g.diag0 + iter % ring];
 		ds->kinetic += k;
 		ds->px += x;
 		ds->py += y;
@@ -1064,12 +1060,24 @@ void k_gather(PushArgs a)
 
 // Places leavers in their destination bucket, or back in the reserved
 // slots of their source bucket when the destination is full.  The last
-// kernel of a step: it also advances the device step counter.
+// kernel of a step: it also advances the device step counter, and with diag
+// adds the push's kinetic and momentum accumulators into each plasma's
+// diagnostic slot (of the iteration the push recorded, so not racing the
+// counter's advance).
 template <typename T, bool IDS, bool PE>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_append(const Geo *__restrict__ geo, const int *__restrict__ pre_bucket, int P,
-		Buckets bk, Parts p, Leavers lv, int par, Counters *ctr, StepState *st)
+		Buckets bk, Parts p, Leavers lv, int par, Counters *ctr, StepState *st,
+		double *__restrict__ dpart, const long long *__restrict__ diag_iter,
+		DiagSlot *__restrict__ diag, int ring)
 {
+	if(diag && threadIdx.x < 64)
+	{
+		const long long it = *diag_iter;
+		if(it >= 0)
+			for(int pl = blockIdx.x; pl < P; pl += gridDim.x)
+				diag_fold(geo[pl], pl, dpart, diag, ring, it, threadIdx.x);
+	}
 	int64_t n = lv.count[par];
 	if(n > lv.cap)
 		n = lv.cap;
@@ -1558,6 +1566,7 @@ push_args(cpic_dev *d, int first, double *rho)
 	a.rho = rho;
 	a.diag = (d->flags & CPIC_F_NO_DIAG) ? (DiagSlot *) NULL : d->diag;
 	a.dpart = d->dpart;
+	a.diag_iter = (long long *) (d->dpart + (size_t) d->P * DIAG_SPREAD * DIAG_LINE);
 	a.ring = d->ring;
 	a.st = d->st;
 	a.derr = d->derr;
@@ -1608,19 +1617,12 @@ cpic_parts_push(cpic_dev *d, int first_iter)
 		DISPATCH_U(k_push, float, dim3(d->nb), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
-	if(a.diag)
-	{
-		slot = cpic_kt_begin(d, "diag_fold");
-		hipLaunchKernelGGL(k_diag_fold, dim3(d->P), dim3(64), 0, d->stream, d->geo, d->dpart,
-				a.diag, d->ring, d->st);
-		cpic_kt_end(d, slot);
-		HIPCHK(hipGetLastError());
-	}
 	d->pcur = 1 - d->pcur;
 
 	slot = cpic_kt_begin(d, "append_leavers");
 	DISPATCH_P(k_append, dim3(512), dim3(CPIC_BLOCK), 0, d->stream, d->geo, PRE(d, PRE_BUCKET),
-			d->P, d->bk, d->p[d->pcur], d->lv, d->step_parity, d->ctr, d->st);
+			d->P, d->bk, d->p[d->pcur], d->lv, d->step_parity, d->ctr, d->st, d->dpart,
+			(const long long *) a.diag_iter, a.diag, d->ring);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	d->step_parity ^= 1;
