@@ -176,7 +176,9 @@ def tune_ensemble(plist=None, seed=2024, steps=20, warmup=6, candidates=CANDIDAT
         ppt_rates[p] = rate(current, p)
         log(f"autotune ensemble: tiles {current} ppt {p}: {ppt_rates[p]:.4g} particle-steps/s")
     ppt = max(ppt_rates, key=ppt_rates.get)
-    mix = (f"ensemble mix: {len(plist)} plasmas (every 4th of the 256-plasma workload), "
+    every = 256 // len(plist) if len(plist) else 0
+    which = "all" if every == 1 else f"every {every}th"
+    mix = (f"ensemble mix: {len(plist)} plasmas ({which} of the 256-plasma workload), "
            f"{ntot} particles, grids {sorted(edges)}")
     for g in edges:
         table[f"fp32/{g}"] = {
@@ -198,10 +200,12 @@ def main():
     ap.add_argument("--out", default=CACHE)
     ap.add_argument("--ensemble", action="store_true",
                     help="tune fp32 tiles and particles per thread on the ensemble mix")
+    ap.add_argument("--every", type=int, default=4,
+                    help="with --ensemble: tune on every N-th plasma of the 256 (1: all)")
     args = ap.parse_args()
     lg = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
     if args.ensemble:
-        tune_ensemble(path=args.out, log=lg)
+        tune_ensemble(plist=ensemble_sample(args.every), path=args.out, log=lg)
     else:
         tune(args.grids, args.precision, args.ppc, path=args.out, log=lg)
     print(json.dumps(load(args.out)))
