@@ -660,7 +660,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 // 2048^2, 64 ppc: the FP32 kernel gains ~3 % from 6 waves (U = 2, now 73
 // VGPRs; 7 waves gain nothing), the FP64 kernel loses from any bound (its
 // state is twice the size); bounding the FP32 U = 4 kernel (then 95 VGPRs,
-// now 91) to 6 waves spilled and lost 12 %.
+// now 93) to 6 waves spilled and lost 12 %.
 #ifndef PUSH_MIN_WAVES_F32
 #define PUSH_MIN_WAVES_F32 6
 #endif
@@ -718,7 +718,7 @@ void k_push(PushArgs a)
 	// one (tile, species) bucket per block: the tile's species run as
 	// separate blocks (fewer chunks per block than one block per tile: a
 	// stream of this access pattern measured 1.60 ms at 8 chunks per block
-	// and 1.55 ms at 4; the push 3 % faster in FP32, 3.5 % in FP64)
+	// and 1.55 ms at 4; the push 2.4 % faster in FP32, 3.4 % in FP64)
 	const int unit = a.order[blockIdx.x];
 	const int pl = find_plasma(a.pre_bucket, a.P, unit);
 	// the plasma's table entry, read once into registers (a reference into
