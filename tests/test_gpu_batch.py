@@ -67,20 +67,30 @@ def make_oracle(c):
     return o
 
 
-def oracle_run(o, nsteps):
+def oracle_run(o, nsteps, kinetic=None):
+    """Field-energy history; with a list `kinetic`, also appends each step's
+    kinetic diagnostic, sum over species of m/4 (|u_old|^2 + |u_new|^2)."""
     o.pre_step()
     W = []
+    ns = o.nspecies
     for it in range(nsteps):
         o.stage("field_E")
         W.append(o.field_energy())
         o.stage("plasma_E")
+        if kinetic is not None:
+            u0 = [o.particles(s) for s in range(ns)]
         o.stage("plasma_r")
+        if kinetic is not None:
+            u1 = [o.particles(s) for s in range(ns)]
+            kinetic.append(sum(0.25 * o.params.m[s] * (np.sum(a["ux"] ** 2 + a["uy"] ** 2) +
+                                                       np.sum(b["ux"] ** 2 + b["uy"] ** 2))
+                               for s, (a, b) in enumerate(zip(u0, u1))))
         o.stage("field_rho")
         o.iter = it + 1
     return np.array(W)
 
 
-def check_plasma(b, p, c, o, W, tol):
+def check_plasma(b, p, c, o, W, tol, K=None):
     for f in ("rho", "phi", "Ex", "Ey"):
         assert rel_l2(b.field(f, plasma=p), o.field(f)) < tol, (p, f)
     for s in range(len(c["q"])):
@@ -92,6 +102,9 @@ def check_plasma(b, p, c, o, W, tol):
     e = b.energy(plasma=p)
     assert len(e["field"]) == len(W)
     assert rel_l2(e["field"], W) < max(tol, 1e-12), (p, rel_l2(e["field"], W))
+    if K is not None:
+        # the push's per-plasma accumulators, summed in k_append
+        assert rel_l2(e["kinetic"], K) < max(tol, 1e-12), (p, rel_l2(e["kinetic"], K))
 
 
 @pytest.mark.parametrize("graphs,particle_e", [(True, True), (False, True), (True, False)])
@@ -107,8 +120,9 @@ def test_batch_fp64_each_plasma_matches_oracle(graphs, particle_e):
     b.sync()
     for p, c in enumerate(cfgs):
         o = make_oracle(c)
-        W = oracle_run(o, 20)
-        check_plasma(b, p, c, o, W, 1e-12)
+        K = []
+        W = oracle_run(o, 20, kinetic=K)
+        check_plasma(b, p, c, o, W, 1e-12, K=K)
 
 
 def test_batch_fp32_each_plasma_close_to_oracle():
