@@ -465,7 +465,7 @@ template <typename T>
 static __device__ __forceinline__ void tile_setup(const Geo &g, int ox, int oy,
 		const typename V2<T>::t *__restrict__ E, typename V2<T>::t *sE, double *sR)
 {
-	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	for(int i = threadIdx.x; i < g.W * g.H; i += blockDim.x)
 	{
 		int ly = i / g.W, lx = i - ly * g.W;
 		int gx = wrapi(ox + lx, g.nx), gy = wrapi(oy + ly, g.ny);
@@ -487,14 +487,14 @@ static __device__ __forceinline__ void tile_setup_phi(const Geo &g, int ox, int 
 {
 	typedef typename V2<T>::t T2;
 	const int PW = g.W + 2, PH = g.H + 2;
-	for(int i = threadIdx.x; i < PW * PH; i += CPIC_BLOCK)
+	for(int i = threadIdx.x; i < PW * PH; i += blockDim.x)
 	{
 		int ly = i / PW, lx = i - ly * PW;
 		int gx = (ox - 1 + lx) & (g.nx - 1), gy = (oy - 1 + ly) & (g.ny - 1);
 		sR[i] = phi[(int64_t) gy * g.nx + gx];
 	}
 	__syncthreads();
-	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	for(int i = threadIdx.x; i < g.W * g.H; i += blockDim.x)
 	{
 		int ly = i / g.W, lx = i - ly * g.W;
 		int c = (ly + 1) * PW + lx + 1;
@@ -506,7 +506,7 @@ static __device__ __forceinline__ void tile_setup_phi(const Geo &g, int ox, int 
 		sE[i] = v;
 	}
 	__syncthreads();
-	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	for(int i = threadIdx.x; i < g.W * g.H; i += blockDim.x)
 		sR[i] = 0.0;
 }
 
@@ -515,7 +515,7 @@ template <typename T>
 static __device__ __forceinline__ void tile_flush(const Geo &g, int ox, int oy, const double *sR,
 		double *__restrict__ rho)
 {
-	for(int i = threadIdx.x; i < g.W * g.H; i += CPIC_BLOCK)
+	for(int i = threadIdx.x; i < g.W * g.H; i += blockDim.x)
 	{
 		double v = sR[i];
 		if(v != 0.0)
@@ -595,16 +595,16 @@ static __device__ __forceinline__ bool push_one(const Geo &g, const typename V2<
 	return true;
 }
 
-// Loads the chunk of U*CPIC_BLOCK particles starting at bucket index base:
-// thread tid takes base + U*tid + u with vector loads when U = 4 and the
-// chunk lies in the bucket's first segment, else base + u*CPIC_BLOCK + tid
+// Loads the chunk of U*NT particles starting at bucket index base (NT
+// threads): thread tid takes base + U*tid + u with vector loads when U = 4
+// and the chunk lies in the bucket's first segment, else base + u*NT + tid
 // (coalesced loads either way; every slot of the chunk exactly once).
-template <typename T, bool IDS, int U>
+template <typename T, bool IDS, int U, int NT>
 static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart, int n1,
 		int seg2, int ntot, int base, int tid, PState<T> *q, bool *act)
 {
 	const GLB_AS T *pb = bucket_base<T>(p, bstart);
-	if(U == 4 && base + U * CPIC_BLOCK <= n1)
+	if(U == 4 && base + U * NT <= n1)
 	{
 		// the whole chunk lies in the first segment (the usual case): thread
 		// tid takes the U consecutive slots base + U*tid ... (U divides the
@@ -631,7 +631,7 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 #pragma unroll
 	for(int u = 0; u < U; u++)
 	{
-		int j = base + u * CPIC_BLOCK + tid;
+		int j = base + u * NT + tid;
 		act[u] = j < ntot;
 		int jj = j + (j >= n1 ? seg2 : 0);
 		if(act[u])
@@ -663,6 +663,12 @@ static __device__ __forceinline__ void load_chunk(const Parts &p, int64_t bstart
 // now 93) to 6 waves spilled and lost 12 %.
 #ifndef PUSH_MIN_WAVES_F32
 #define PUSH_MIN_WAVES_F32 6
+#endif
+// Threads per k_push workgroup.  512 (half the chunks per bucket; the
+// compiler then holds the FP32 kernel to 81 VGPRs) measured slower at
+// 2048^2: FP32 push 1.89 vs 1.74 ms, FP64 3.78 vs 3.29 ms.
+#ifndef PUSH_BLOCK
+#define PUSH_BLOCK CPIC_BLOCK
 #endif
 
 
@@ -706,7 +712,7 @@ static __device__ __forceinline__ void back_to_tile(const Geo &g, int ndst, int 
 
 // EPHI: the tile's E comes from the potential (the E array is stale).
 template <typename T, bool IDS, bool PE, int U, bool EPHI>
-__global__ __launch_bounds__(CPIC_BLOCK, sizeof(T) == 4 && U == 2 && !IDS && !PE ? PUSH_MIN_WAVES_F32 : 1)
+__global__ __launch_bounds__(PUSH_BLOCK, sizeof(T) == 4 && U == 2 && !IDS && !PE ? PUSH_MIN_WAVES_F32 : 1)
 void k_push(PushArgs a)
 {
 	typedef typename V2<T>::t T2;
@@ -767,12 +773,12 @@ void k_push(PushArgs a)
 	const int seg2 = cap - res - n1;      // slot = j + seg2 for j >= n1
 	const SpeciesK<T> sk = species_k<T>(*gp, s, dtp);
 
-	for(int base = 0; base < ntot; base += U * CPIC_BLOCK)
+	for(int base = 0; base < ntot; base += U * PUSH_BLOCK)
 	{
 		PState<T> q[U];
 		bool act[U], leave[U];
 		int ndst[U];
-		load_chunk<T, IDS, U>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
+		load_chunk<T, IDS, U, PUSH_BLOCK>(a.p, bstart, n1, seg2, ntot, base, tid, q, act);
 #pragma unroll
 		for(int u = 0; u < U; u++)
 		{
@@ -905,7 +911,7 @@ void k_push(PushArgs a)
 	{
 		const int nst = s_lv.n;
 		const int64_t lb0 = s_lv.base;
-		for(int k = tid; k < nst; k += CPIC_BLOCK)
+		for(int k = tid; k < nst; k += PUSH_BLOCK)
 		{
 			PState<T> lq;
 			lq.x = s_lv.x[k];
@@ -956,7 +962,7 @@ void k_push(PushArgs a)
 		mpy = wave_sum_lane63(mpy);
 		if(lane == 63)
 		{
-			int slot = (blockIdx.x * (CPIC_BLOCK / 64) + wid) & (DIAG_SPREAD - 1);
+			int slot = (blockIdx.x * (PUSH_BLOCK / 64) + wid) & (DIAG_SPREAD - 1);
 			double *acc = a.dpart + ((int64_t) pl * DIAG_SPREAD + slot) * DIAG_LINE;
 			unsafeAtomicAdd(&acc[0], ke);
 			unsafeAtomicAdd(&acc[1], mpx);
@@ -1612,9 +1618,9 @@ cpic_parts_push(cpic_dev *d, int first_iter)
 	a.order = d->border;
 	slot = cpic_kt_begin(d, "push_gather_deposit");
 	if(d->prec == 64)
-		DISPATCH_U(k_push, double, dim3(d->nb), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
+		DISPATCH_U(k_push, double, dim3(d->nb), dim3(PUSH_BLOCK), d->lds_push, d->stream, a);
 	else
-		DISPATCH_U(k_push, float, dim3(d->nb), dim3(CPIC_BLOCK), d->lds_push, d->stream, a);
+		DISPATCH_U(k_push, float, dim3(d->nb), dim3(PUSH_BLOCK), d->lds_push, d->stream, a);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 	d->pcur = 1 - d->pcur;
