@@ -336,6 +336,34 @@ def test_failed_resort_leaves_a_handle_that_refuses_work(monkeypatch):
     d.close()
 
 
+def test_bucket_over_the_slot_limit_refused():
+    """One (tile, species) bucket holds at most 2^27 slots (the push
+    addresses a bucket with 32-bit byte offsets): a commit whose bucket would
+    need more (110 M particles in one 4x4 tile, 1.25x capacity) returns
+    CPIC_ENOMEM before it changes the particle storage, and the handle keeps
+    working with an upload that fits."""
+    d = ca.Device(4, 4, (4.0, 4.0), 0.1, [-1.0], [1.0], precision=32, tile=(4, 4))
+    rng = np.random.default_rng(21)
+    n = 110_000_000
+    empty = np.empty(0, np.int64)
+    x = rng.uniform(0.0, 4.0, n)
+    y = rng.uniform(0.0, 4.0, n)
+    u = np.zeros(n)
+    d.upload(0, empty, x, y, u, u)
+    del x, y
+    with pytest.raises(ca.CpicError) as ei:
+        d.pre_step()
+    assert ei.value.code == -2
+    k = 1000
+    d.upload(0, empty, rng.uniform(0.0, 4.0, k), rng.uniform(0.0, 4.0, k), np.zeros(k),
+             np.zeros(k))
+    d.pre_step()
+    d.run(0, 3)
+    d.sync()
+    assert len(d.particles(0)["x"]) == k
+    d.close()
+
+
 def test_in_plane_B_rejected():
     with pytest.raises(ca.CpicError) as ei:
         ca.Device(16, 16, (16.0, 16.0), 0.1, [-1.0], [1.0], B=(0.0, 1.0, 0.0))
