@@ -13,8 +13,9 @@
 //                   (transposed) so that R consecutive ky are contiguous.
 //   2. k_cols       one or more kx columns per workgroup: forward FFT along
 //                   ky, multiply by G/(nx*ny) (the normalisation is folded in;
-//                   nx*ny is a power of two so the scaling is exact), the field
-//                   energy partial sum, inverse FFT along ky.
+//                   nx*ny is a power of two so the scaling is exact) and the
+//                   field energy partial sum as the last forward stage writes
+//                   LDS, inverse FFT along ky.
 //   3. k_rows_c2r   R rows per workgroup: Hermitian pre-processing + inverse
 //                   half-length FFT; writes phi rows, and zeroes the same rows
 //                   of the rho buffer the next push deposits into and the next
@@ -28,7 +29,8 @@
 // Twiddles: each workgroup copies two small factor tables into LDS from one
 // half table of exp(+2 pi i k/N), N = the largest grid edge of the batch,
 // built with an octant reduction so the quarter points are exact; a
-// twiddle is then the product of two LDS entries.  The block size of each
+// butterfly's twiddle w is then the product of two LDS entries and its
+// other powers w^r are products of w.  The block size of each
 // pass is chosen per batch (cpic_fft_threads).  Each pass runs once for all
 // plasmas of a batch: block b takes the b-th item of the pass's work list
 // (row block or column block) and finds its plasma by binary search.
@@ -160,18 +162,48 @@ static __device__ __forceinline__ void dft_small(double2 *v, double sign)
 #define FFT_MIN_WAVES 8
 static_assert(FFT_MAXV * FFT_THREADS >= FFT_MAX_VALUES, "FFT stage capacity");
 
+// Multiplies v[r], r = 1 .. R-1, by w^r for a butterfly's twiddle w: one
+// table lookup per butterfly instead of R-1, the powers by products at
+// most three deep (w^2, w^4 = (w^2)^2, w^3 = w^2 w, w^5 = w^4 w,
+// w^6 = w^4 w^2, w^7 = w^4 w^3), each within a few ulp of the exact root.
+template <int R>
+static __device__ __forceinline__ void apply_twiddles(double2 *v, double2 w1)
+{
+	v[1] = cmul(v[1], w1);
+	if(R >= 4)
+	{
+		double2 w2 = cmul(w1, w1);
+		v[2] = cmul(v[2], w2);
+		double2 w3 = cmul(w2, w1);
+		v[3] = cmul(v[3], w3);
+		if(R == 8)
+		{
+			double2 w4 = cmul(w2, w2);
+			v[4] = cmul(v[4], w4);
+			v[5] = cmul(v[5], cmul(w4, w1));
+			v[6] = cmul(v[6], cmul(w4, w2));
+			v[7] = cmul(v[7], cmul(w4, w3));
+		}
+	}
+}
+
+struct NoPost {
+	__device__ double2 operator()(int, int, double2 v) const { return v; }
+};
+
 // One Stockham stage of radix R over nbat transforms of length n = 2^lgn,
 // current span p = 2^lgp: butterfly i of a transform reads x[i + t n/R],
 // applies w^(t k) with w = exp(sign 2 pi i/(R p)), k = i mod p, and writes
 // the R-point DFT to y[(i - k) R + k + t p].  x comes from the LDS buffer
 // (transform b at b*stride, IN_LDS) or from in(b, j); y goes to the LDS
-// buffer (OUT_LDS) or to out(b, j, value).  In place in LDS, all reads
-// precede a barrier and all writes follow it, so one buffer holds both; a
-// stage that writes LDS ends with a barrier.  Needs (n/R)*nbat <=
-// (FFT_MAXV/R) * NT.
-template <int R, int NT, bool IN_LDS, bool OUT_LDS, class In, class Out>
+// buffer (OUT_LDS, as post(b, j, value)) or to out(b, j, value).  In place
+// in LDS, all reads precede a barrier and all writes follow it, so one
+// buffer holds both; a stage that writes LDS ends with a barrier.  Needs
+// (n/R)*nbat <= (FFT_MAXV/R) * NT.
+template <int R, int NT, bool IN_LDS, bool OUT_LDS, class In, class Out, class Post = NoPost>
 static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, int lgn,
-		int nbat, int lgp, const TwRef &tw, int lgN, double sign, In in, Out out)
+		int nbat, int lgp, const TwRef &tw, int lgN, double sign, In in, Out out,
+		Post post = Post())
 {
 	constexpr int LGR = R == 8 ? 3 : R == 4 ? 2 : 1;
 	constexpr int MAXB = FFT_MAXV / R;
@@ -195,9 +227,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 				v[m][r] = IN_LDS ? buf[bat * stride + i + r * nb] : in(bat, i + r * nb);
 			if(k)
 			{
-#pragma unroll
-				for(int r = 1; r < R; r++)
-					v[m][r] = cmul(v[m][r], twiddle_at(tw, (r * k) << tsh, sign));
+				apply_twiddles<R>(v[m], twiddle_at(tw, k << tsh, sign));
 			}
 			dft_small<R>(v[m], sign);
 			jo[m] = ((i - k) << LGR) + k;
@@ -214,7 +244,7 @@ static __device__ __forceinline__ void stockham_stage(double2 *buf, int stride, 
 			for(int r = 0; r < R; r++)
 			{
 				if(OUT_LDS)
-					buf[bo[m] * stride + jo[m] + r * p] = v[m][r];
+					buf[bo[m] * stride + jo[m] + r * p] = post(bo[m], jo[m] + r * p, v[m][r]);
 				else
 					out(bo[m], jo[m] + r * p, v[m][r]);
 			}
@@ -239,30 +269,46 @@ static __device__ __forceinline__ int first_lgr(int lgn)
 
 // FFT of nbat transforms of length n = 2^lgn (Stockham autosort, natural
 // order in and out), w = exp(sign 2 pi i/len), reading element j of
-// transform b through in(b, j) and leaving the result in the LDS buffer
-// (transform b at b*stride).  Ends at a barrier.
-template <int NT, class In>
+// transform b through in(b, j) and leaving post(b, j, X_j) in the LDS
+// buffer (transform b at b*stride).  Ends at a barrier.
+template <int NT, class In, class Post = NoPost>
 static __device__ void fft_from(double2 *buf, int stride, int lgn, int nbat, const TwRef &tw,
-		int lgN, double sign, In in)
+		int lgN, double sign, In in, Post post = Post())
 {
 	NoOut none;
 	if(lgn == 0)
 	{
 		for(int b = threadIdx.x; b < nbat; b += NT)
-			buf[b * stride] = in(b, 0);
+			buf[b * stride] = post(b, 0, in(b, 0));
 		__syncthreads();
 		return;
 	}
-	int lgp = first_lgr(lgn);
-	if(lgp == 1)
+	int l0 = first_lgr(lgn);
+	if(l0 == lgn)
+	{
+		if(l0 == 1)
+			stockham_stage<2, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none,
+					post);
+		else if(l0 == 2)
+			stockham_stage<4, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none,
+					post);
+		else
+			stockham_stage<8, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none,
+					post);
+		return;
+	}
+	if(l0 == 1)
 		stockham_stage<2, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
-	else if(lgp == 2)
+	else if(l0 == 2)
 		stockham_stage<4, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
 	else
 		stockham_stage<8, NT, false, true>(buf, stride, lgn, nbat, 0, tw, lgN, sign, in, none);
-	for(; lgp < lgn; lgp += 3)
+	int lgp = l0;
+	for(; lgp + 3 < lgn; lgp += 3)
 		stockham_stage<8, NT, true, true>(buf, stride, lgn, nbat, lgp, tw, lgN, sign,
 				NoIn(), none);
+	stockham_stage<8, NT, true, true>(buf, stride, lgn, nbat, lgp, tw, lgN, sign, NoIn(), none,
+			post);
 }
 
 // The same transform from the LDS buffer (written before a barrier), its
@@ -410,56 +456,71 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
 
 	// the first stage reads the columns straight from ghat
-	fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, [=](int c, int y) {
+	auto load = [=](int c, int y) {
 		return (packed && c == 0) ? make_double2(ghat[y].x, ghat[(int64_t) m * ny + y].x) :
 			ghat[(int64_t) (k0 + c) * ny + y];
-	});
-
+	};
 	// G exactly as MFT_init computes it (src/solver.c:269-280)
 	auto green = [&](int kx, int ky) {
 		return (kx == 0 && ky == 0) ? 0.0 : 1.0 / (2.0 * (cosx[kx] + cosy[ky]) - 4.0);
 	};
 	double e = 0.0;
-	// the packed column's ky pairs (ky, ny - ky) first (one thread per pair),
-	// then the other columns' values
-	const int npair = packed ? (ny >> 1) + 1 : 0;
-	for(int j = threadIdx.x; j < npair + (ncol - (packed ? 1 : 0)) * ny; j += blockDim.x)
+	if(!packed)
 	{
-		if(j < npair)
+		// G/(nx ny) applied and the energy summed as the last forward stage
+		// writes each value (one LDS round trip and barrier fewer)
+		fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, load, [&](int c, int ky, double2 v) {
+			double G = green(k0 + c, ky);
+			e += 2.0 * G * (v.x * v.x + v.y * v.y);
+			double gs = G * invN;
+			return make_double2(v.x * gs, v.y * gs);
+		});
+	}
+	else
+	{
+		fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, load);
+		// the packed column's ky pairs (ky, ny - ky) first (one thread per pair),
+		// then the other columns' values
+		const int npair = (ny >> 1) + 1;
+		for(int j = threadIdx.x; j < npair + (ncol - 1) * ny; j += blockDim.x)
 		{
-			// Z = A + iB with A, B the spectra of the real columns kx = 0
-			// and kx = m: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] -
-			// conj Z[-k])/(2i), A[-k] = conj A[k], B[-k] = conj B[k]
-			int k = j, kn = (ny - j) & (ny - 1);
-			double2 zk = sh[k], zn = sh[kn];
-			double2 A = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
-			double2 B = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
-			double a2 = A.x * A.x + A.y * A.y, b2 = B.x * B.x + B.y * B.y;
-			double ga = green(0, k), gb = green(m, k);
-			e += ga * a2 + gb * b2;
-			ga *= invN;
-			gb *= invN;
-			// Z'[k] = ga A + i gb B
-			sh[k] = make_double2(ga * A.x - gb * B.y, ga * A.y + gb * B.x);
-			if(kn != k)
+			if(j < npair)
 			{
-				double gan = green(0, kn), gbn = green(m, kn);
-				e += gan * a2 + gbn * b2;
-				gan *= invN;
-				gbn *= invN;
-				// Z'[-k] = gan conj(A) + i gbn conj(B)
-				sh[kn] = make_double2(gan * A.x + gbn * B.y, -gan * A.y + gbn * B.x);
+				// Z = A + iB with A, B the spectra of the real columns kx = 0
+				// and kx = m: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] -
+				// conj Z[-k])/(2i), A[-k] = conj A[k], B[-k] = conj B[k]
+				int k = j, kn = (ny - j) & (ny - 1);
+				double2 zk = sh[k], zn = sh[kn];
+				double2 A = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+				double2 B = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+				double a2 = A.x * A.x + A.y * A.y, b2 = B.x * B.x + B.y * B.y;
+				double ga = green(0, k), gb = green(m, k);
+				e += ga * a2 + gb * b2;
+				ga *= invN;
+				gb *= invN;
+				// Z'[k] = ga A + i gb B
+				sh[k] = make_double2(ga * A.x - gb * B.y, ga * A.y + gb * B.x);
+				if(kn != k)
+				{
+					double gan = green(0, kn), gbn = green(m, kn);
+					e += gan * a2 + gbn * b2;
+					gan *= invN;
+					gbn *= invN;
+					// Z'[-k] = gan conj(A) + i gbn conj(B)
+					sh[kn] = make_double2(gan * A.x + gbn * B.y, -gan * A.y + gbn * B.x);
+				}
+				continue;
 			}
-			continue;
+			int i = j - npair + ny;
+			int c = i >> lgny, ky = i & (ny - 1);
+			int kx = k0 + c;
+			double G = green(kx, ky);
+			double2 v = sh[i];
+			e += 2.0 * G * (v.x * v.x + v.y * v.y);
+			double g = G * invN;
+			sh[i] = make_double2(v.x * g, v.y * g);
 		}
-		int i = j - npair + (packed ? ny : 0);
-		int c = i >> lgny, ky = i & (ny - 1);
-		int kx = k0 + c;
-		double G = green(kx, ky);
-		double2 v = sh[i];
-		e += 2.0 * G * (v.x * v.x + v.y * v.y);
-		double g = G * invN;
-		sh[i] = make_double2(v.x * g, v.y * g);
+		__syncthreads();
 	}
 	if(energy)
 	{
@@ -477,7 +538,6 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 			unsafeAtomicAdd(energy, t * escale);
 		}
 	}
-	__syncthreads();
 	// the last stage writes the columns straight to ghat
 	fft_to<NT>(sh, ny, lgny, ncol, twr, lgN, +1.0, [=](int c, int y, double2 v) {
 		if(packed && c == 0)
