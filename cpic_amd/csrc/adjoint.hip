@@ -30,8 +30,6 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define NRED 256          // partial sums of the field-energy reduction
-
 struct AdjSpecies {
 	double h[CPIC_MAX_SPECIES];   // 0.5 dt' q / m
 	double tz[CPIC_MAX_SPECIES];  // Bz h
@@ -65,7 +63,6 @@ struct cpic_adj {
 	hipEvent_t ev[4];         // device work of the last forward [0,1] and backward [2,3]
 	int evok;                 // events created
 	int evrec;                // bit 0: forward pair recorded, bit 1: backward pair
-	double *part;             // NRED partial sums
 	int *derr;
 	int64_t nsteps;
 	int K;
@@ -363,45 +360,6 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 	box_flush(g, b, sacc, 1, rho_next, 1);
 }
 
-// part[b] = sum over block b's contiguous range of a*b (fixed order)
-__global__ __launch_bounds__(CPIC_BLOCK)
-void k_adj_dot(const double *__restrict__ a, const double *__restrict__ b, int64_t n,
-		double *__restrict__ part)
-{
-	__shared__ double red[CPIC_BLOCK];
-	int64_t per = (n + gridDim.x - 1) / gridDim.x;
-	int64_t i0 = blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
-	double s = 0.0;
-	for(int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
-		s += a[i] * b[i];
-	red[threadIdx.x] = s;
-	__syncthreads();
-	for(int w = CPIC_BLOCK / 2; w > 0; w >>= 1)
-	{
-		if((int) threadIdx.x < w)
-			red[threadIdx.x] += red[threadIdx.x + w];
-		__syncthreads();
-	}
-	if(threadIdx.x == 0)
-		part[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(NRED)
-void k_adj_sum(const double *__restrict__ part, double scale, double *__restrict__ out)
-{
-	__shared__ double red[NRED];
-	red[threadIdx.x] = part[threadIdx.x];
-	__syncthreads();
-	for(int w = NRED / 2; w > 0; w >>= 1)
-	{
-		if((int) threadIdx.x < w)
-			red[threadIdx.x] += red[threadIdx.x + w];
-		__syncthreads();
-	}
-	if(threadIdx.x == 0)
-		*out = scale * red[0];
-}
-
 // Reverse of move + Boris + gather for one iteration; accumulates the
 // gather-weight derivative into (lx, ly) and deposits lE_p into lE.
 // REV_PPT particles per thread: the kernel keeps six values per particle
@@ -516,12 +474,17 @@ void k_adj_field(Geo g, const double2 *__restrict__ lE, const double *__restrict
 	}
 }
 
-// lr += c * grad Gat(lrho)(r)
+// lr += c * grad Gat(lrho)(r); also zeroes lE (nz doubles), which this
+// iteration's k_adj_field has read and the next k_adj_rev_a deposits into
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_rev_b(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ sp,
 		const double *__restrict__ x, const double *__restrict__ y,
-		const double *__restrict__ lrho, double *__restrict__ lx, double *__restrict__ ly)
+		const double *__restrict__ lrho, double *__restrict__ lx, double *__restrict__ ly,
+		double *__restrict__ lE, int64_t nz)
 {
+	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < nz;
+			i += (int64_t) gridDim.x * blockDim.x)
+		lE[i] = 0.0;
 	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
 			i += (int64_t) gridDim.x * blockDim.x)
 	{
@@ -565,7 +528,7 @@ adj_free(cpic_adj *a)
 	hipFree(a->rho[0]); hipFree(a->rho[1]); hipFree(a->phi); hipFree(a->psi); hipFree(a->lrho);
 	hipFree(a->E); hipFree(a->lE);
 	hipFree(a->lx); hipFree(a->ly); hipFree(a->lux); hipFree(a->luy);
-	hipFree(a->ck); hipFree(a->seg); hipFree(a->segf); hipFree(a->hist); hipFree(a->part);
+	hipFree(a->ck); hipFree(a->seg); hipFree(a->segf); hipFree(a->hist);
 	hipFree(a->derr);
 	hipFree(a->wk);
 	if(a->evok)
@@ -635,7 +598,6 @@ cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
 			hipMalloc(&a->phi, gn) != hipSuccess || hipMalloc(&a->psi, gn) != hipSuccess ||
 			hipMalloc(&a->lrho, gn) != hipSuccess ||
 			hipMalloc(&a->E, 2 * gn) != hipSuccess || hipMalloc(&a->lE, 2 * gn) != hipSuccess ||
-			hipMalloc(&a->part, sizeof(double) * NRED) != hipSuccess ||
 			hipMalloc(&a->derr, sizeof(int)) != hipSuccess)
 	{
 		adj_free(a);
@@ -707,23 +669,17 @@ deposit(cpic_adj *a, const double *x, const double *y, double *rho)
 	return 0;
 }
 
-// phi and E (into E) from rho; with w, the field energy of rho into *w;
-// with rho_zero, that (other) buffer is zeroed by the solve's last pass
+// phi and E (into E) from rho; with w, the field energy of rho,
+// -(e0/2) sum rho phi, is added to *w (zero beforehand; the column pass of
+// the solve sums it over the spectrum, sum_k G |rho^(k)|^2 / (nx ny), equal
+// to sum rho phi by Parseval); with rho_zero, that (other) buffer is zeroed
+// by the solve's last pass
 static int
 fields(cpic_adj *a, const double *rho, double *w, double *rho_zero, double2 *E)
 {
 	int ret;
-	int64_t nodes = (int64_t) a->g.nx * a->g.ny;
-	if((ret = cpic_solve_ptr_zero(a->d, rho, a->phi, rho_zero)))
+	if((ret = cpic_solve_ptr_energy(a->d, rho, a->phi, rho_zero, w)))
 		return ret;
-	if(w)
-	{
-		hipLaunchKernelGGL(k_adj_dot, dim3(NRED), dim3(CPIC_BLOCK), 0, a->d->stream, rho, a->phi,
-				nodes, a->part);
-		hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(NRED), 0, a->d->stream, a->part,
-				-0.5 * a->g.e0, w);
-		HIPCHK(hipGetLastError());
-	}
 	return cpic_efield_ptr(a->d, a->phi, E);
 }
 
@@ -863,7 +819,7 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	a->have = 0;
 	a->evrec = 0;             // a backward belongs to the forward before it
 	HIPCHK(hipEventRecord(a->ev[0], a->d->stream));
-	if((ret = deposit(a, a->x, a->y, a->rho[0])))
+	if((ret = deposit(a, a->x, a->y, a->rho[0])) || (ret = zero(a, a->hist, nsteps)))
 		return ret;
 	for(int64_t it = 0; it < nsteps; it++)
 	{
@@ -952,8 +908,9 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	}
 	a->evrec &= ~2;
 	HIPCHK(hipEventRecord(a->ev[2], a->d->stream));
+	// lE starts at zero; each iteration's k_adj_rev_b zeroes it for the next
 	if((ret = zero(a, a->lx, n)) || (ret = zero(a, a->ly, n)) || (ret = zero(a, a->lux, n)) ||
-			(ret = zero(a, a->luy, n)))
+			(ret = zero(a, a->luy, n)) || (ret = zero(a, (double *) a->lE, 2 * nodes)))
 		return ret;
 	// keep the forward's final state: the recompute runs on copies
 	double *fx = a->x, *fy = a->y, *fux = a->ux, *fuy = a->uy;
@@ -1002,8 +959,6 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			const double *sx = a->seg + 2 * n * (it - it0), *sy = sx + n;
 			int first = it == 0;
 			double dtp = first ? -a->g.dt / 2 : a->g.dt;
-			if((ret = zero(a, (double *) a->lE, 2 * nodes)))
-				break;
 			hipLaunchKernelGGL(k_adj_rev_a, dim3((int) ((n + REV_CHUNK - 1) / REV_CHUNK)),
 					dim3(CPIC_BLOCK), 0, a->d->stream,
 					a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
@@ -1019,7 +974,8 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			if((ret = cpic_solve_ptr(a->d, a->psi, a->lrho)))
 				break;
 			hipLaunchKernelGGL(k_adj_rev_b, dim3(pblocks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
-					a->g, a->k[1], n, a->sp, sx, sy, a->lrho, a->lx, a->ly);
+					a->g, a->k[1], n, a->sp, sx, sy, a->lrho, a->lx, a->ly, (double *) a->lE,
+					2 * nodes);
 			if(hipGetLastError() != hipSuccess)
 				ret = CPIC_EHIP;
 		}

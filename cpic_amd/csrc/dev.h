@@ -233,6 +233,10 @@ int cpic_solve_ptr(cpic_dev *d, const double *rho, double *phi);
 // the same, also zeroing the nodes of rho_zero (another buffer) in the
 // inverse row pass
 int cpic_solve_ptr_zero(cpic_dev *d, const double *rho, double *phi, double *rho_zero);
+// the same, also adding each plasma's field energy -(e0/2) sum rho phi to
+// energy[p] (summed over the spectrum by the column pass)
+int cpic_solve_ptr_energy(cpic_dev *d, const double *rho, double *phi, double *rho_zero,
+		double *energy);
 int cpic_efield_ptr(cpic_dev *d, const double *phi, void *E);
 int cpic_efield(cpic_dev *d);
 int cpic_parts_init(cpic_dev *d);

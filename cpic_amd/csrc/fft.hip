@@ -434,7 +434,7 @@ __global__ __launch_bounds__(NT, FFT_MIN_WAVES)
 void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		double2 *__restrict__ ghat_all, const double *__restrict__ costab_all,
 		const double2 *__restrict__ tw, int lgN, DiagSlot *__restrict__ diag,
-		int ring, const StepState *__restrict__ st)
+		int ring, const StepState *__restrict__ st, double *__restrict__ eout)
 {
 	extern __shared__ double2 sh[];
 	__shared__ double red[NT / 64];
@@ -450,7 +450,9 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 	const double *cosx = costab_all + g.cos0, *cosy = cosx + (m + 1);
 	double invN = 1.0 / ((double) nx * (double) ny);
 	long long iter = st->iter;
-	double *energy = (diag && iter >= 0) ? &diag[g.diag0 + iter % ring].field : (double *) NULL;
+	// the field energy goes to eout[p] when given, else to the step's slot
+	double *energy = eout ? eout + p :
+		(diag && iter >= 0) ? &diag[g.diag0 + iter % ring].field : (double *) NULL;
 	double escale = -0.5 * g.e0 * invN;
 	__shared__ TwLds s_tw;
 	const TwRef twr = tw_fill(s_tw, tw, lgN);
@@ -897,10 +899,10 @@ cpic_fft_blocking_ok(const Geo *g, int nt_row, int nt_col)
 
 // phi = S rho for every plasma of the batch (rho and phi laid out like the
 // state's own buffers); with diag, the field energy of each plasma goes to
-// its slot of the current step.
+// its slot of the current step, with eout, it is added to eout[p].
 static int
 solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag, double *rho_zero,
-		DiagSlot *diag_zero)
+		DiagSlot *diag_zero, double *eout = NULL)
 {
 	int slot;
 	const int P = d->P;
@@ -915,7 +917,7 @@ solve(cpic_dev *d, const double *rho, double *phi, DiagSlot *diag, double *rho_z
 	slot = cpic_kt_begin(d, "fft_cols_green");
 	FFT_LAUNCH(k_cols, d->fft_nt_col, d->colblocks, d->lds_cols, d->stream,
 			d->geo, pre + PRE_COL * (P + 1), P, d->ghat, d->costab, d->tw, d->lgN, diag,
-			d->ring, d->st);
+			d->ring, d->st, eout);
 	cpic_kt_end(d, slot);
 	HIPCHK(hipGetLastError());
 
@@ -962,6 +964,13 @@ int
 cpic_solve_ptr_zero(cpic_dev *d, const double *rho, double *phi, double *rho_zero)
 {
 	return solve(d, rho, phi, (DiagSlot *) NULL, rho_zero, (DiagSlot *) NULL);
+}
+
+int
+cpic_solve_ptr_energy(cpic_dev *d, const double *rho, double *phi, double *rho_zero,
+		double *energy)
+{
+	return solve(d, rho, phi, (DiagSlot *) NULL, rho_zero, (DiagSlot *) NULL, energy);
 }
 
 int
@@ -1157,7 +1166,7 @@ cpic_solve_xy_noalloc(int64_t nx, int64_t ny, const double *rho, double *phi, vo
 	HIPCHK(hipGetLastError());
 	FFT_LAUNCH(k_cols, nt_col, cols, lds_cols, s,
 			geo, pre + 2, 1, ghat, (const double *) costab, (const double2 *) tw, lgN,
-			(DiagSlot *) NULL, 1, st);
+			(DiagSlot *) NULL, 1, st, (double *) NULL);
 	HIPCHK(hipGetLastError());
 	FFT_LAUNCH(k_rows_c2r, nt_row, rows, lds_rows, s,
 			geo, pre, 1, (const double2 *) ghat, phi, (const double2 *) tw, lgN,
