@@ -49,7 +49,7 @@ struct cpic_adj {
 	double *hbuf;             // host staging, 4n doubles (pinned)
 	double *x, *y, *ux, *uy;  // state
 	double *rho[2], *phi, *psi, *lrho;
-	double2 *E, *lE;
+	double2 *E, *lE;          // lE: two buffers of nodes values (reverse iterations alternate)
 	double *lx, *ly, *lux, *luy;
 	double *ck;               // checkpoints, 4n doubles each
 	int64_t ckcap;
@@ -360,8 +360,25 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 	box_flush(g, b, sacc, 1, rho_next, 1);
 }
 
+// (ax, ay) += c grad Gat(lrho)(x, y): the derivative of the CIC interpolant
+// of lrho at a particle's position, the last term of lr
+static __device__ __forceinline__ void rev_b_term(const Geo &g, double c, double x, double y,
+		const double *__restrict__ lrho, double &ax, double &ay)
+{
+	int i00, i01, i10, i11;
+	double rx, ry;
+	cic_nodes(g, x, y, i00, i01, i10, i11, rx, ry);
+	double l00 = lrho[i00], l01 = lrho[i01], l10 = lrho[i10], l11 = lrho[i11];
+	double dxw = 1.0 - rx, dyw = 1.0 - ry;
+	ax += c * (g.idx * (dyw * (l10 - l00) + ry * (l11 - l01)));
+	ay += c * (g.idy * (dxw * (l01 - l00) + rx * (l11 - l10)));
+}
+
 // Reverse of move + Boris + gather for one iteration; accumulates the
-// gather-weight derivative into (lx, ly) and deposits lE_p into lE.
+// gather-weight derivative into (lx, ly) and deposits lE_p into lE.  With
+// PEND, first adds the lrho term of the iteration after it (rev_b_term at
+// that iteration's positions (bx, by) with its lrho), so that (lx, ly) are
+// read and written once for both.
 // REV_PPT particles per thread: the kernel keeps six values per particle
 // across the box barriers, and 8 per thread would hold it to 3 waves per
 // SIMD.  A fraction of a round-robin chunk still spans its cells.
@@ -371,12 +388,14 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 #define REV_CHUNK (REV_PPT * CPIC_BLOCK)
 static_assert(ADJ_CHUNK % REV_CHUNK == 0, "reverse blocks split the sort's chunks");
 #define REV_BOX_NODES 1536          // 2 components: 24 KB
+template <bool PEND>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		const int32_t *__restrict__ sp, const double *__restrict__ x,
 		const double *__restrict__ y, const double2 *__restrict__ E, double *__restrict__ lx,
 		double *__restrict__ ly, double *__restrict__ lux, double *__restrict__ luy,
-		double2 *__restrict__ lE)
+		double2 *__restrict__ lE, const double *__restrict__ bx, const double *__restrict__ by,
+		const double *__restrict__ lrho)
 {
 	__shared__ double sacc[2 * REV_BOX_NODES];
 	__shared__ int s6[6];
@@ -393,6 +412,8 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		int s = sp[i];
 		double mx = lux[i], my = luy[i];
 		double ax = lx[i], ay = ly[i];
+		if(PEND)
+			rev_b_term(g, k.c[s], bx[i], by[i], lrho, ax, ay);
 		if(set_r)
 		{
 			mx = fma(dtp, ax, mx);
@@ -454,10 +475,12 @@ void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 }
 
 // psi = D^T lE - srho * rho, D the central-difference E stencil
-// (E_x[i] = (phi[i-1] - phi[i+1]) / 2dx, so (D_x^T a)[i] = (a[i+1] - a[i-1]) / 2dx)
+// (E_x[i] = (phi[i-1] - phi[i+1]) / 2dx, so (D_x^T a)[i] = (a[i+1] - a[i-1]) / 2dx);
+// zeroes lE_zero (the other lE buffer, which the next reverse iteration
+// deposits into)
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_field(Geo g, const double2 *__restrict__ lE, const double *__restrict__ rho,
-		double srho, double *__restrict__ psi)
+		double srho, double *__restrict__ psi, double2 *__restrict__ lE_zero)
 {
 	int64_t n = (int64_t) g.nx * g.ny;
 	double dx2 = 2 * g.dx, dy2 = 2 * g.dy;
@@ -471,31 +494,24 @@ void k_adj_field(Geo g, const double2 *__restrict__ lE, const double *__restrict
 		double lphi = (lE[row + xp].x - lE[row + xm].x) / dx2 +
 			(lE[(int64_t) yp * g.nx + ix].y - lE[(int64_t) ym * g.nx + ix].y) / dy2;
 		psi[i] = lphi - srho * rho[i];
+		lE_zero[i] = make_double2(0.0, 0.0);
 	}
 }
 
-// lr += c * grad Gat(lrho)(r); also zeroes lE (nz doubles), which this
-// iteration's k_adj_field has read and the next k_adj_rev_a deposits into
+// lr += c * grad Gat(lrho)(r) for the last iteration of the sweep (the
+// others are applied by the next k_adj_rev_a)
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_rev_b(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ sp,
 		const double *__restrict__ x, const double *__restrict__ y,
-		const double *__restrict__ lrho, double *__restrict__ lx, double *__restrict__ ly,
-		double *__restrict__ lE, int64_t nz)
+		const double *__restrict__ lrho, double *__restrict__ lx, double *__restrict__ ly)
 {
-	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < nz;
-			i += (int64_t) gridDim.x * blockDim.x)
-		lE[i] = 0.0;
 	for(int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n;
 			i += (int64_t) gridDim.x * blockDim.x)
 	{
-		int i00, i01, i10, i11;
-		double rx, ry;
-		cic_nodes(g, x[i], y[i], i00, i01, i10, i11, rx, ry);
-		double c = k.c[sp[i]];
-		double l00 = lrho[i00], l01 = lrho[i01], l10 = lrho[i10], l11 = lrho[i11];
-		double dxw = 1.0 - rx, dyw = 1.0 - ry;
-		lx[i] += c * (g.idx * (dyw * (l10 - l00) + ry * (l11 - l01)));
-		ly[i] += c * (g.idy * (dxw * (l01 - l00) + rx * (l11 - l10)));
+		double ax = lx[i], ay = ly[i];
+		rev_b_term(g, k.c[sp[i]], x[i], y[i], lrho, ax, ay);
+		lx[i] = ax;
+		ly[i] = ay;
 	}
 }
 
@@ -597,7 +613,7 @@ cpic_adj_create(const cpic_dev_desc_t *desc, int64_t n, const int32_t *species,
 			hipMalloc(&a->rho[0], gn) != hipSuccess || hipMalloc(&a->rho[1], gn) != hipSuccess ||
 			hipMalloc(&a->phi, gn) != hipSuccess || hipMalloc(&a->psi, gn) != hipSuccess ||
 			hipMalloc(&a->lrho, gn) != hipSuccess ||
-			hipMalloc(&a->E, 2 * gn) != hipSuccess || hipMalloc(&a->lE, 2 * gn) != hipSuccess ||
+			hipMalloc(&a->E, 2 * gn) != hipSuccess || hipMalloc(&a->lE, 4 * gn) != hipSuccess ||
 			hipMalloc(&a->derr, sizeof(int)) != hipSuccess)
 	{
 		adj_free(a);
@@ -908,9 +924,10 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	}
 	a->evrec &= ~2;
 	HIPCHK(hipEventRecord(a->ev[2], a->d->stream));
-	// lE starts at zero; each iteration's k_adj_rev_b zeroes it for the next
+	// both lE buffers start at zero; each iteration's k_adj_field zeroes the
+	// one the next iteration deposits into
 	if((ret = zero(a, a->lx, n)) || (ret = zero(a, a->ly, n)) || (ret = zero(a, a->lux, n)) ||
-			(ret = zero(a, a->luy, n)) || (ret = zero(a, (double *) a->lE, 2 * nodes)))
+			(ret = zero(a, a->luy, n)) || (ret = zero(a, (double *) a->lE, 4 * nodes)))
 		return ret;
 	// keep the forward's final state: the recompute runs on copies
 	double *fx = a->x, *fy = a->y, *fux = a->ux, *fuy = a->uy;
@@ -959,13 +976,26 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			const double *sx = a->seg + 2 * n * (it - it0), *sy = sx + n;
 			int first = it == 0;
 			double dtp = first ? -a->g.dt / 2 : a->g.dt;
-			hipLaunchKernelGGL(k_adj_rev_a, dim3((int) ((n + REV_CHUNK - 1) / REV_CHUNK)),
-					dim3(CPIC_BLOCK), 0, a->d->stream,
-					a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
-					(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, a->lE);
+			double2 *lE = a->lE + (it & 1) * nodes, *lE_next = a->lE + ((it + 1) & 1) * nodes;
+			dim3 rg((unsigned) ((n + REV_CHUNK - 1) / REV_CHUNK));
+			if(it + 1 < T)
+			{
+				// the lrho term of iteration it + 1, at its positions: the
+				// segment's next slot, or the next checkpoint's
+				const double *bx = it + 1 < it1 ? sx + 2 * n : a->ck + 4 * n * (sg + 1);
+				hipLaunchKernelGGL(k_adj_rev_a<true>, rg, dim3(CPIC_BLOCK), 0, a->d->stream,
+						a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
+						(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, lE,
+						bx, bx + n, (const double *) a->lrho);
+			}
+			else
+				hipLaunchKernelGGL(k_adj_rev_a<false>, rg, dim3(CPIC_BLOCK), 0, a->d->stream,
+						a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
+						(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, lE,
+						(const double *) NULL, (const double *) NULL, (const double *) NULL);
 			hipLaunchKernelGGL(k_adj_field, dim3(pblocks(nodes)), dim3(CPIC_BLOCK), 0,
-					a->d->stream, a->g, (const double2 *) a->lE, seg_rho(it - it0),
-					dloss * a->g.e0, a->psi);
+					a->d->stream, a->g, (const double2 *) lE, seg_rho(it - it0),
+					dloss * a->g.e0, a->psi, lE_next);
 			if(hipGetLastError() != hipSuccess)
 			{
 				ret = CPIC_EHIP;
@@ -973,12 +1003,16 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			}
 			if((ret = cpic_solve_ptr(a->d, a->psi, a->lrho)))
 				break;
-			hipLaunchKernelGGL(k_adj_rev_b, dim3(pblocks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
-					a->g, a->k[1], n, a->sp, sx, sy, a->lrho, a->lx, a->ly, (double *) a->lE,
-					2 * nodes);
-			if(hipGetLastError() != hipSuccess)
-				ret = CPIC_EHIP;
 		}
+	}
+	// the lrho term of iteration 0 (positions: segment 0's first slot)
+	if(!ret)
+	{
+		hipLaunchKernelGGL(k_adj_rev_b, dim3(pblocks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+				a->g, a->k[1], n, a->sp, (const double *) a->seg, (const double *) a->seg + n,
+				(const double *) a->lrho, a->lx, a->ly);
+		if(hipGetLastError() != hipSuccess)
+			ret = CPIC_EHIP;
 	}
 	if(!ret && hipEventRecord(a->ev[3], a->d->stream) == hipSuccess)
 		a->evrec |= 2;
