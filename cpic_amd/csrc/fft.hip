@@ -467,10 +467,14 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		return (kx == 0 && ky == 0) ? 0.0 : 1.0 / (2.0 * (cosx[kx] + cosy[ky]) - 4.0);
 	};
 	double e = 0.0;
-	if(!packed)
+	// butterflies of the last forward stage (radix 8 from 8 points up)
+	const int lastb = ncol * (ny >= 8 ? ny / 8 : 1);
+	if(!packed && lastb >= NT)
 	{
 		// G/(nx ny) applied and the energy summed as the last forward stage
-		// writes each value (one LDS round trip and barrier fewer)
+		// writes each value (one LDS round trip and barrier fewer); only
+		// when that stage keeps every thread busy, as the Green's function
+		// costs a division per value
 		fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, load, [&](int c, int ky, double2 v) {
 			double G = green(k0 + c, ky);
 			e += 2.0 * G * (v.x * v.x + v.y * v.y);
@@ -483,8 +487,8 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 		fft_from<NT>(sh, ny, lgny, ncol, twr, lgN, -1.0, load);
 		// the packed column's ky pairs (ky, ny - ky) first (one thread per pair),
 		// then the other columns' values
-		const int npair = (ny >> 1) + 1;
-		for(int j = threadIdx.x; j < npair + (ncol - 1) * ny; j += blockDim.x)
+		const int npair = packed ? (ny >> 1) + 1 : 0;
+		for(int j = threadIdx.x; j < npair + (ncol - (packed ? 1 : 0)) * ny; j += blockDim.x)
 		{
 			if(j < npair)
 			{
@@ -513,7 +517,7 @@ void k_cols(const Geo *__restrict__ geo, const int *__restrict__ pre, int P,
 				}
 				continue;
 			}
-			int i = j - npair + ny;
+			int i = j - npair + (packed ? ny : 0);
 			int c = i >> lgny, ky = i & (ny - 1);
 			int kx = k0 + c;
 			double G = green(kx, ky);
