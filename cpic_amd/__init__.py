@@ -35,7 +35,7 @@ EXPORTS = (
     "cpic_batch_upload_particles", "cpic_batch_download_particles", "cpic_batch_count",
     "cpic_batch_download_field", "cpic_batch_upload_field", "cpic_batch_energy",
     "cpic_batch_cell_counts", "cpic_adj_create", "cpic_adj_destroy", "cpic_adj_forward",
-    "cpic_adj_backward", "cpic_adj_state", "cpic_adj_device_ms",
+    "cpic_adj_backward", "cpic_adj_state", "cpic_adj_device_ms", "cpic_adj_set_field_history",
 )
 
 
@@ -126,6 +126,7 @@ def lib():
         L.cpic_adj_backward.argtypes = [vp, C.c_double, d, d, d, d]
         L.cpic_adj_state.argtypes = [vp, d, d, d, d]
         L.cpic_adj_device_ms.argtypes = [vp, d, d]
+        L.cpic_adj_set_field_history.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 

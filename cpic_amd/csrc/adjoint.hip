@@ -59,6 +59,13 @@ struct cpic_adj {
 	int64_t segfcap;
 	double *hist;             // W per iteration
 	int64_t histcap;
+	// field history of the last forward (when it fits ADJ_FIELD_HISTORY_MAX
+	// and is not switched off): rho of iterations 0..T and E of 0..T-1,
+	// which the reverse sweep reads instead of recomputing them
+	double *frho, *fE;
+	int64_t frhocap, fEcap;   // doubles
+	int fhist;                // the last forward kept it
+	int fhist_off;            // cpic_adj_set_field_history(a, 0)
 	double *wk;               // recompute state of the reverse sweep, 4n doubles
 	hipEvent_t ev[4];         // device work of the last forward [0,1] and backward [2,3]
 	int evok;                 // events created
@@ -283,9 +290,10 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 	box_flush(g, b, sacc, 1, rho, 1);
 }
 
-// gather + Boris + move + wrap, then deposit at the new position into
+// gather + Boris + move + wrap, then (DEP) deposit at the new position into
 // rho_next; positions are read from (x, y) and written to (xo, yo), which may
 // be the same arrays
+template <bool DEP>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		const int32_t *__restrict__ sp, const double *x, const double *y, double *xo,
@@ -348,6 +356,8 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		atomicOr(derr, CPIC_DERR_UMAX);
 	if(nan_)
 		atomicOr(derr, CPIC_DERR_NAN);
+	if(!DEP)
+		return;
 	Box b = block_box<ADJ_PPT>(g, ix, iy, s6, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
@@ -388,6 +398,10 @@ static __device__ __forceinline__ void rev_b_term(const Geo &g, double c, double
 #define REV_CHUNK (REV_PPT * CPIC_BLOCK)
 static_assert(ADJ_CHUNK % REV_CHUNK == 0, "reverse blocks split the sort's chunks");
 #define REV_BOX_NODES 1536          // 2 components: 24 KB
+// Device memory the field history of a forward may take (rho and E of every
+// iteration, 24 B per node and iteration: 315 MB for 200 iterations at
+// 256^2); beyond it the reverse sweep recomputes each segment's fields
+#define ADJ_FIELD_HISTORY_MAX ((int64_t) 8 << 30)
 template <bool PEND>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_rev_a(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
@@ -545,6 +559,7 @@ adj_free(cpic_adj *a)
 	hipFree(a->E); hipFree(a->lE);
 	hipFree(a->lx); hipFree(a->ly); hipFree(a->lux); hipFree(a->luy);
 	hipFree(a->ck); hipFree(a->seg); hipFree(a->segf); hipFree(a->hist);
+	hipFree(a->frho); hipFree(a->fE);
 	hipFree(a->derr);
 	hipFree(a->wk);
 	if(a->evok)
@@ -711,9 +726,25 @@ forward_step(cpic_adj *a, int64_t it, const double *x, const double *y, double *
 		return ret;
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
-	hipLaunchKernelGGL(k_adj_push, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+	hipLaunchKernelGGL(k_adj_push<true>, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
 			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, x, y, xo, yo, a->ux,
 			a->uy, (const double2 *) E, rho_out, a->derr);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// The particle half of forward_step, for the reverse sweep's recompute with
+// a field history: positions (and the velocities in place) of iteration
+// it + 1 from those of it with E of it; no deposit
+static int
+move_step(cpic_adj *a, int64_t it, const double *x, const double *y, double *xo, double *yo,
+		const double2 *E)
+{
+	int first = it == 0;
+	double dtp = first ? -a->g.dt / 2 : a->g.dt;
+	hipLaunchKernelGGL(k_adj_push<false>, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, x, y, xo, yo, a->ux,
+			a->uy, E, (double *) NULL, a->derr);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -834,8 +865,24 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 	HIPCHK(hipMemsetAsync(a->derr, 0, sizeof(int), a->d->stream));
 	a->have = 0;
 	a->evrec = 0;             // a backward belongs to the forward before it
+	// the field history, when it fits: each iteration's rho and E are
+	// written straight into their slots (the solve zeroes the slot the push
+	// deposits into), at no extra work in the forward
+	const int64_t nodes = (int64_t) a->g.nx * a->g.ny;
+	a->fhist = 0;
+	if(!a->fhist_off &&
+			(double) nodes * (3.0 * (double) nsteps + 1.0) * sizeof(double) <=
+			(double) ADJ_FIELD_HISTORY_MAX)
+	{
+		if(!grow(&a->frho, &a->frhocap, nodes * (nsteps + 1)) &&
+				!grow(&a->fE, &a->fEcap, 2 * nodes * nsteps))
+			a->fhist = 1;
+		else
+			(void) hipGetLastError();   // no room: the reverse recomputes instead
+	}
 	HIPCHK(hipEventRecord(a->ev[0], a->d->stream));
-	if((ret = deposit(a, a->x, a->y, a->rho[0])) || (ret = zero(a, a->hist, nsteps)))
+	if((ret = deposit(a, a->x, a->y, a->fhist ? a->frho : a->rho[0])) ||
+			(ret = zero(a, a->hist, nsteps)))
 		return ret;
 	for(int64_t it = 0; it < nsteps; it++)
 	{
@@ -843,7 +890,12 @@ cpic_adj_forward(cpic_adj_t *a, const double *x, const double *y, const double *
 			if((ret = copy_state(a, a->ck + 4 * n * (it / K), a->x, a->y, a->ux, a->uy,
 					hipMemcpyDeviceToDevice)))
 				return ret;
-		if((ret = forward_iter(a, it, &cur, a->hist + it)))
+		if(a->fhist)
+			ret = forward_step(a, it, a->x, a->y, a->x, a->y, a->frho + nodes * it,
+					a->frho + nodes * (it + 1), (double2 *) a->fE + nodes * it, a->hist + it);
+		else
+			ret = forward_iter(a, it, &cur, a->hist + it);
+		if(ret)
 			return ret;
 	}
 	HIPCHK(hipEventRecord(a->ev[1], a->d->stream));
@@ -892,6 +944,15 @@ cpic_adj_device_ms(cpic_adj_t *a, double *forward_ms, double *backward_ms)
 }
 
 extern "C" int
+cpic_adj_set_field_history(cpic_adj_t *a, int on)
+{
+	if(!a)
+		return CPIC_EINVAL;
+	a->fhist_off = !on;
+	return 0;
+}
+
+extern "C" int
 cpic_adj_state(cpic_adj_t *a, double *x, double *y, double *ux, double *uy)
 {
 	if(!a || !a->have)
@@ -914,8 +975,9 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	const int64_t nck = (T + K - 1) / K;
 	// a segment holds at most min(K, T) iterations (the last one T - (nck-1) K)
 	const int64_t Kseg = K < T ? K : T;
+	const int fh = a->fhist;
 	if((ret = grow(&a->seg, &a->segcap, 2 * n * Kseg)) ||
-			(ret = grow(&a->segf, &a->segfcap, 3 * nodes * Kseg)))
+			(!fh && (ret = grow(&a->segf, &a->segfcap, 3 * nodes * Kseg))))
 		return ret;
 	if(!a->wk && hipMalloc(&a->wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
 	{
@@ -937,11 +999,18 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	a->ux = wk + 2 * n;
 	a->uy = wk + 3 * n;
 	ret = 0;
-	// the segment's rho and E of iteration i (recomputed below, read by the
-	// reverse iteration): no deposit or solve of the forward's fields again
-	double *const segf = a->segf;
-	auto seg_rho = [segf, nodes](int64_t i) { return segf + 3 * nodes * i; };
-	auto seg_E = [segf, nodes](int64_t i) { return (double2 *) (segf + 3 * nodes * i + nodes); };
+	// rho and E of iteration it (in the segment starting at it0), read by
+	// the reverse iteration: the forward's field history, or recomputed with
+	// the segment's positions below (no deposit or solve again in the
+	// reverse iteration either way)
+	double *const segf = a->segf, *const frho = a->frho;
+	double2 *const fE = (double2 *) a->fE;
+	auto seg_rho = [=](int64_t it, int64_t it0) {
+		return fh ? frho + nodes * it : segf + 3 * nodes * (it - it0);
+	};
+	auto seg_E = [=](int64_t it, int64_t it0) {
+		return fh ? fE + nodes * it : (double2 *) (segf + 3 * nodes * (it - it0) + nodes);
+	};
 	for(int64_t sg = nck - 1; sg >= 0 && !ret; sg--)
 	{
 		int64_t it0 = sg * K, it1 = it0 + K < T ? it0 + K : T;
@@ -959,17 +1028,30 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			ret = CPIC_EHIP;
 			break;
 		}
-		if((ret = deposit(a, a->seg, a->seg + n, seg_rho(0))))
-			break;
-		for(int64_t it = it0; it < it1 && !ret; it++)
+		if(fh)
 		{
-			double *sx = a->seg + 2 * n * (it - it0);
-			if(it + 1 < it1)
-				ret = forward_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n, seg_rho(it - it0),
-						seg_rho(it + 1 - it0), seg_E(it - it0), (double *) NULL);
-			else
-				ret = fields(a, seg_rho(it - it0), (double *) NULL, (double *) NULL,
-						seg_E(it - it0));
+			// positions only, moved by the history's E
+			for(int64_t it = it0; it + 1 < it1 && !ret; it++)
+			{
+				double *sx = a->seg + 2 * n * (it - it0);
+				ret = move_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n, seg_E(it, it0));
+			}
+		}
+		else
+		{
+			if((ret = deposit(a, a->seg, a->seg + n, seg_rho(it0, it0))))
+				break;
+			for(int64_t it = it0; it < it1 && !ret; it++)
+			{
+				double *sx = a->seg + 2 * n * (it - it0);
+				if(it + 1 < it1)
+					ret = forward_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n,
+							seg_rho(it, it0), seg_rho(it + 1, it0), seg_E(it, it0),
+							(double *) NULL);
+				else
+					ret = fields(a, seg_rho(it, it0), (double *) NULL, (double *) NULL,
+							seg_E(it, it0));
+			}
 		}
 		for(int64_t it = it1 - 1; it >= it0 && !ret; it--)
 		{
@@ -985,16 +1067,16 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 				const double *bx = it + 1 < it1 ? sx + 2 * n : a->ck + 4 * n * (sg + 1);
 				hipLaunchKernelGGL(k_adj_rev_a<true>, rg, dim3(CPIC_BLOCK), 0, a->d->stream,
 						a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
-						(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, lE,
+						(const double2 *) seg_E(it, it0), a->lx, a->ly, a->lux, a->luy, lE,
 						bx, bx + n, (const double *) a->lrho);
 			}
 			else
 				hipLaunchKernelGGL(k_adj_rev_a<false>, rg, dim3(CPIC_BLOCK), 0, a->d->stream,
 						a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
-						(const double2 *) seg_E(it - it0), a->lx, a->ly, a->lux, a->luy, lE,
+						(const double2 *) seg_E(it, it0), a->lx, a->ly, a->lux, a->luy, lE,
 						(const double *) NULL, (const double *) NULL, (const double *) NULL);
 			hipLaunchKernelGGL(k_adj_field, dim3(pblocks(nodes)), dim3(CPIC_BLOCK), 0,
-					a->d->stream, a->g, (const double2 *) lE, seg_rho(it - it0),
+					a->d->stream, a->g, (const double2 *) lE, seg_rho(it, it0),
 					dloss * a->g.e0, a->psi, lE_next);
 			if(hipGetLastError() != hipSuccess)
 			{

@@ -65,6 +65,11 @@ class AdjointRun:
         ca.check(ca.lib().cpic_adj_state(self._h, *(ca._dp(a) for a in out)))
         return out
 
+    def set_field_history(self, on):
+        """Whether later forwards keep every iteration's rho and E (when they
+        fit) for the backward, instead of its recomputing them."""
+        ca.check(ca.lib().cpic_adj_set_field_history(self._h, 1 if on else 0))
+
     def device_ms(self):
         """Device time of the last forward and backward in ms (-1 if not run)."""
         f, b = C.c_double(), C.c_double()

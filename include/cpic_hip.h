@@ -259,8 +259,11 @@ int cpic_batch_cell_counts(cpic_dev_t *dev, int32_t plasma, int32_t species,
  * W = -(e0/2) sum rho*phi  (the value cpic_dev_energy reports per step),
  * with respect to every particle's initial position and velocity.  Forward
  * states are checkpointed every `checkpoint_every` iterations and the
- * reverse sweep recomputes each segment from its checkpoint.  desc must
- * have precision 64; flags, tile and slack are ignored.
+ * reverse sweep recomputes each segment's positions from its checkpoint;
+ * the fields of every iteration (rho and E, 24 bytes per node and
+ * iteration) are kept by the forward when they fit 8 GiB of device memory
+ * (cpic_adj_set_field_history), else recomputed with the positions.  desc
+ * must have precision 64; flags, tile and slack are ignored.
  */
 typedef struct cpic_adj cpic_adj_t;
 
@@ -289,6 +292,13 @@ int cpic_adj_state(cpic_adj_t *adj, double *x, double *y, double *ux, double *uy
  * -1 for one that has not run.  No counterpart in the reference, which has
  * no differentiable run. */
 int cpic_adj_device_ms(cpic_adj_t *adj, double *forward_ms, double *backward_ms);
+
+/* on = 0: later forwards keep no field history, and the backward after such
+ * a forward recomputes each segment's rho and E (less device memory, more
+ * work); on = 1 (the default): keep it when it fits.  The backward follows
+ * what its forward did.  No counterpart in the reference, which has no
+ * differentiable run. */
+int cpic_adj_set_field_history(cpic_adj_t *adj, int on);
 
 #ifdef __cplusplus
 }

@@ -107,6 +107,17 @@ def test_adjoint_gradient_matches_torch_reference_64():
         for a, b, gr in zip(g1, gK, grefs):
             assert rel_l2(b, a) < 1e-9, K
             assert rel_l2(b, gr) < 1e-7, K
+    # without the forward's field history the reverse sweep recomputes each
+    # segment's rho and E with its positions: the same gradients
+    run.set_field_history(False)
+    for K in (20, 30):
+        lossK, _ = run.forward(*c["st"], 200, K)
+        gK = run.backward(1.0)
+        assert abs(lossK - float(lref)) < 1e-10 * abs(float(lref))
+        for a, b, gr in zip(g1, gK, grefs):
+            assert rel_l2(b, a) < 1e-9, K
+            assert rel_l2(b, gr) < 1e-7, K
+    run.set_field_history(True)
     run.close()
 
 
