@@ -53,7 +53,7 @@ struct cpic_adj {
 	double *lx, *ly, *lux, *luy;
 	double *ck;               // checkpoints, 4n doubles each
 	int64_t ckcap;
-	double *seg;              // positions of one segment's iterations, 2n doubles each
+	double *seg;              // positions of one segment's iterations after its first, 2n doubles each
 	int64_t segcap;
 	double *segf;             // rho and E of one segment's iterations, 3 nodes doubles each
 	int64_t segfcap;
@@ -973,10 +973,11 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	hipSetDevice(a->d->device);
 	const int64_t n = a->n, K = a->K, T = a->nsteps, nodes = (int64_t) a->g.nx * a->g.ny;
 	const int64_t nck = (T + K - 1) / K;
-	// a segment holds at most min(K, T) iterations (the last one T - (nck-1) K)
+	// a segment holds at most min(K, T) iterations (the last one T - (nck-1) K),
+	// the positions of all but its first in the segment buffer
 	const int64_t Kseg = K < T ? K : T;
 	const int fh = a->fhist;
-	if((ret = grow(&a->seg, &a->segcap, 2 * n * Kseg)) ||
+	if((ret = grow(&a->seg, &a->segcap, 2 * n * (Kseg - 1))) ||
 			(!fh && (ret = grow(&a->segf, &a->segfcap, 3 * nodes * Kseg))))
 		return ret;
 	if(!a->wk && hipMalloc(&a->wk, sizeof(double) * 4 * (size_t) n) != hipSuccess)
@@ -1014,16 +1015,19 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 	for(int64_t sg = nck - 1; sg >= 0 && !ret; sg--)
 	{
 		int64_t it0 = sg * K, it1 = it0 + K < T ? it0 + K : T;
-		// positions of iteration i live in the segment's slot i: the
-		// checkpoint's go to slot 0, each push writes the next slot
+		// positions (x, then y) of iteration it: the checkpoint's for the
+		// segment's first, slot it - it0 - 1 of the segment buffer, which the
+		// recompute's pushes write, for the others
 		const double *c = a->ck + 4 * n * sg;
-		if((ret = copy_state(a, a->seg, c, c + n, (const double *) NULL, (const double *) NULL,
-				hipMemcpyDeviceToDevice)))
-			break;
-		if(hipMemcpyAsync(a->ux, c + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
-					a->d->stream) != hipSuccess ||
+		auto pos = [&](int64_t it) -> double * {
+			return it == it0 ? (double *) c : a->seg + 2 * n * (it - it0 - 1);
+		};
+		// velocities only for a segment that moves its particles
+		if(it1 - it0 > 1 &&
+				(hipMemcpyAsync(a->ux, c + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+						a->d->stream) != hipSuccess ||
 				hipMemcpyAsync(a->uy, c + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
-					a->d->stream) != hipSuccess)
+						a->d->stream) != hipSuccess))
 		{
 			ret = CPIC_EHIP;
 			break;
@@ -1032,20 +1036,17 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 		{
 			// positions only, moved by the history's E
 			for(int64_t it = it0; it + 1 < it1 && !ret; it++)
-			{
-				double *sx = a->seg + 2 * n * (it - it0);
-				ret = move_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n, seg_E(it, it0));
-			}
+				ret = move_step(a, it, pos(it), pos(it) + n, pos(it + 1), pos(it + 1) + n,
+						seg_E(it, it0));
 		}
 		else
 		{
-			if((ret = deposit(a, a->seg, a->seg + n, seg_rho(it0, it0))))
+			if((ret = deposit(a, pos(it0), pos(it0) + n, seg_rho(it0, it0))))
 				break;
 			for(int64_t it = it0; it < it1 && !ret; it++)
 			{
-				double *sx = a->seg + 2 * n * (it - it0);
 				if(it + 1 < it1)
-					ret = forward_step(a, it, sx, sx + n, sx + 2 * n, sx + 3 * n,
+					ret = forward_step(a, it, pos(it), pos(it) + n, pos(it + 1), pos(it + 1) + n,
 							seg_rho(it, it0), seg_rho(it + 1, it0), seg_E(it, it0),
 							(double *) NULL);
 				else
@@ -1055,7 +1056,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 		}
 		for(int64_t it = it1 - 1; it >= it0 && !ret; it--)
 		{
-			const double *sx = a->seg + 2 * n * (it - it0), *sy = sx + n;
+			const double *sx = pos(it), *sy = sx + n;
 			int first = it == 0;
 			double dtp = first ? -a->g.dt / 2 : a->g.dt;
 			double2 *lE = a->lE + (it & 1) * nodes, *lE_next = a->lE + ((it + 1) & 1) * nodes;
@@ -1064,7 +1065,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			{
 				// the lrho term of iteration it + 1, at its positions: the
 				// segment's next slot, or the next checkpoint's
-				const double *bx = it + 1 < it1 ? sx + 2 * n : a->ck + 4 * n * (sg + 1);
+				const double *bx = it + 1 < it1 ? pos(it + 1) : a->ck + 4 * n * (sg + 1);
 				hipLaunchKernelGGL(k_adj_rev_a<true>, rg, dim3(CPIC_BLOCK), 0, a->d->stream,
 						a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, n, a->sp, sx, sy,
 						(const double2 *) seg_E(it, it0), a->lx, a->ly, a->lux, a->luy, lE,
@@ -1087,11 +1088,11 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 				break;
 		}
 	}
-	// the lrho term of iteration 0 (positions: segment 0's first slot)
+	// the lrho term of iteration 0 (positions: the first checkpoint's)
 	if(!ret)
 	{
 		hipLaunchKernelGGL(k_adj_rev_b, dim3(pblocks(n)), dim3(CPIC_BLOCK), 0, a->d->stream,
-				a->g, a->k[1], n, a->sp, (const double *) a->seg, (const double *) a->seg + n,
+				a->g, a->k[1], n, a->sp, (const double *) a->ck, (const double *) a->ck + n,
 				(const double *) a->lrho, a->lx, a->ly);
 		if(hipGetLastError() != hipSuccess)
 			ret = CPIC_EHIP;
