@@ -102,6 +102,13 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 #define ADJ_PPT 4
 #endif
 #define ADJ_CHUNK (ADJ_PPT * CPIC_BLOCK)
+// Particles per thread of the reverse sweep's recompute push, which only
+// moves particles (no deposit, no LDS box): at 256^2 (4,194,304 particles)
+// 4 per thread make ~2.3 rounds of workgroups, the last a third full;
+// smaller blocks of work keep the chip busier to the end
+#ifndef MOVE_PPT
+#define MOVE_PPT 2
+#endif
 #ifndef ADJ_BOX_NODES
 #define ADJ_BOX_NODES 3072          // 24 KB of double
 #endif
@@ -292,8 +299,9 @@ void k_adj_deposit(Geo g, AdjSpecies k, int64_t n, const int32_t *__restrict__ s
 
 // gather + Boris + move + wrap, then (DEP) deposit at the new position into
 // rho_next; positions are read from (x, y) and written to (xo, yo), which may
-// be the same arrays
-template <bool DEP>
+// be the same arrays.  PPT particles per thread: ADJ_PPT with the deposit
+// (its LDS box spans one sort chunk), MOVE_PPT without
+template <bool DEP, int PPT>
 __global__ __launch_bounds__(CPIC_BLOCK)
 void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		const int32_t *__restrict__ sp, const double *x, const double *y, double *xo,
@@ -303,11 +311,11 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 	__shared__ double sacc[ADJ_BOX_NODES];
 	__shared__ int s6[6];
 	bool bad = false, nan_ = false;
-	int64_t p0 = (int64_t) blockIdx.x * ADJ_CHUNK;
-	int ix[ADJ_PPT], iy[ADJ_PPT];
-	double rx[ADJ_PPT], ry[ADJ_PPT];
+	int64_t p0 = (int64_t) blockIdx.x * (PPT * CPIC_BLOCK);
+	int ix[PPT], iy[PPT];
+	double rx[PPT], ry[PPT];
 #pragma unroll
-	for(int u = 0; u < ADJ_PPT; u++)
+	for(int u = 0; u < PPT; u++)
 	{
 		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
 		ix[u] = -1;
@@ -358,10 +366,10 @@ void k_adj_push(Geo g, AdjSpecies k, int set_r, double dtp, int64_t n,
 		atomicOr(derr, CPIC_DERR_NAN);
 	if(!DEP)
 		return;
-	Box b = block_box<ADJ_PPT>(g, ix, iy, s6, ADJ_BOX_NODES);
+	Box b = block_box<PPT>(g, ix, iy, s6, ADJ_BOX_NODES);
 	box_zero(b, sacc, 1);
 #pragma unroll
-	for(int u = 0; u < ADJ_PPT; u++)
+	for(int u = 0; u < PPT; u++)
 	{
 		int64_t i = p0 + u * CPIC_BLOCK + threadIdx.x;
 		if(ix[u] >= 0)
@@ -391,9 +399,11 @@ static __device__ __forceinline__ void rev_b_term(const Geo &g, double c, double
 // read and written once for both.
 // REV_PPT particles per thread: the kernel keeps six values per particle
 // across the box barriers, and 8 per thread would hold it to 3 waves per
-// SIMD.  A fraction of a round-robin chunk still spans its cells.
+// SIMD; 2 rather than 4 make twice the workgroups, so that at 256^2 the last
+// of their rounds idles less of the chip (reverse sweep 34.4 -> 33.5 ms).
+// A fraction of a round-robin chunk still spans its cells.
 #ifndef REV_PPT
-#define REV_PPT 4
+#define REV_PPT 2
 #endif
 #define REV_CHUNK (REV_PPT * CPIC_BLOCK)
 static_assert(ADJ_CHUNK % REV_CHUNK == 0, "reverse blocks split the sort's chunks");
@@ -726,7 +736,8 @@ forward_step(cpic_adj *a, int64_t it, const double *x, const double *y, double *
 		return ret;
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
-	hipLaunchKernelGGL(k_adj_push<true>, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+	hipLaunchKernelGGL((k_adj_push<true, ADJ_PPT>), dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0,
+			a->d->stream,
 			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, x, y, xo, yo, a->ux,
 			a->uy, (const double2 *) E, rho_out, a->derr);
 	HIPCHK(hipGetLastError());
@@ -742,7 +753,9 @@ move_step(cpic_adj *a, int64_t it, const double *x, const double *y, double *xo,
 {
 	int first = it == 0;
 	double dtp = first ? -a->g.dt / 2 : a->g.dt;
-	hipLaunchKernelGGL(k_adj_push<false>, dim3(chunks(a->n)), dim3(CPIC_BLOCK), 0, a->d->stream,
+	hipLaunchKernelGGL((k_adj_push<false, MOVE_PPT>),
+			dim3((unsigned) ((a->n + MOVE_PPT * CPIC_BLOCK - 1) / (MOVE_PPT * CPIC_BLOCK))),
+			dim3(CPIC_BLOCK), 0, a->d->stream,
 			a->g, a->k[first ? 0 : 1], first ? 0 : 1, dtp, a->n, a->sp, x, y, xo, yo, a->ux,
 			a->uy, E, (double *) NULL, a->derr);
 	HIPCHK(hipGetLastError());
