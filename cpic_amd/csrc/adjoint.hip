@@ -1032,9 +1032,8 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 		// segment's first, slot it - it0 - 1 of the segment buffer, which the
 		// recompute's pushes write, for the others
 		const double *c = a->ck + 4 * n * sg;
-		auto pos = [&](int64_t it) -> double * {
-			return it == it0 ? (double *) c : a->seg + 2 * n * (it - it0 - 1);
-		};
+		auto slot = [&](int64_t it) -> double * { return a->seg + 2 * n * (it - it0 - 1); };
+		auto pos = [&](int64_t it) -> const double * { return it == it0 ? c : slot(it); };
 		// velocities only for a segment that moves its particles
 		if(it1 - it0 > 1 &&
 				(hipMemcpyAsync(a->ux, c + 2 * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
@@ -1049,7 +1048,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 		{
 			// positions only, moved by the history's E
 			for(int64_t it = it0; it + 1 < it1 && !ret; it++)
-				ret = move_step(a, it, pos(it), pos(it) + n, pos(it + 1), pos(it + 1) + n,
+				ret = move_step(a, it, pos(it), pos(it) + n, slot(it + 1), slot(it + 1) + n,
 						seg_E(it, it0));
 		}
 		else
@@ -1059,7 +1058,7 @@ cpic_adj_backward(cpic_adj_t *a, double dloss, double *gx, double *gy, double *g
 			for(int64_t it = it0; it < it1 && !ret; it++)
 			{
 				if(it + 1 < it1)
-					ret = forward_step(a, it, pos(it), pos(it) + n, pos(it + 1), pos(it + 1) + n,
+					ret = forward_step(a, it, pos(it), pos(it) + n, slot(it + 1), slot(it + 1) + n,
 							seg_rho(it, it0), seg_rho(it + 1, it0), seg_E(it, it0),
 							(double *) NULL);
 				else
