@@ -109,8 +109,11 @@ static __device__ __forceinline__ void cic_nodes(const Geo &g, double x, double 
 #ifndef MOVE_PPT
 #define MOVE_PPT 2
 #endif
+// 20 KB of double: seven depositing blocks per CU (their 72 VGPRs allow
+// seven; a 24 KB box held them to six: forward 15.8 -> 15.7 ms at 256^2);
+// the boxes there hold at most ~125 nodes
 #ifndef ADJ_BOX_NODES
-#define ADJ_BOX_NODES 3072          // 24 KB of double
+#define ADJ_BOX_NODES 2560
 #endif
 
 // LDS accumulation box of a block: nodes [x0, x0 + w) x [y0, y0 + h) in
