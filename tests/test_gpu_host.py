@@ -93,7 +93,7 @@ def test_cyclotron_through_host_driver():
     zero only in exact arithmetic: with e0 = 8.85e-12 its rounding residual
     (E ~ 1e-7 at the particle, the same in the oracle) bounds the checks at
     1e-7 relative.  The centre is taken from the orbit itself (the
-    reference's test places it on the wrong side, SURVEY's caveat aside).
+    reference's test computes it on the wrong side of the particle).
     The trajectory also matches the oracle's step by step."""
     import pyoracle as po
     h = host()
