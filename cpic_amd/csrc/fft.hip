@@ -14,8 +14,9 @@
 //   2. k_cols       one or more kx columns per workgroup: forward FFT along
 //                   ky, multiply by G/(nx*ny) (the normalisation is folded in;
 //                   nx*ny is a power of two so the scaling is exact) and the
-//                   field energy partial sum as the last forward stage writes
-//                   LDS, inverse FFT along ky.
+//                   field energy partial sum (as the last forward stage writes
+//                   LDS when that stage fills the workgroup, else in a sweep
+//                   of their own), inverse FFT along ky.
 //   3. k_rows_c2r   R rows per workgroup: Hermitian pre-processing + inverse
 //                   half-length FFT; writes phi rows, and zeroes the same rows
 //                   of the rho buffer the next push deposits into and the next
