@@ -19,9 +19,10 @@
 //   lr = lr' + grad Gat(E)(r) . lE_p + c grad Gat(lrho)(r),  c = -q/e0
 // (the last two are derivatives of the CIC interpolant with respect to the
 // particle position).  States are checkpointed every K iterations in the
-// forward; the reverse recomputes each segment from its checkpoint, storing
-// the positions of its iterations.  No counterpart in the reference, which
-// has no gradients.
+// forward, which also keeps every iteration's rho and E when they fit a
+// fixed budget; the reverse recomputes each segment's positions from its
+// checkpoint (and, without that history, their rho and E).  No counterpart
+// in the reference, which has no gradients.
 #include "dev.h"
 #include "host_sort.h"
 #include "picmath.h"
