@@ -72,3 +72,12 @@ def test_invalid_batches_rejected_before_any_device_call():
     with pytest.raises(ca.CpicError) as ei:
         ca.Batch([])
     assert ei.value.code == -1
+
+
+def test_adjoint_calls_reject_a_null_handle_before_any_device_call():
+    """The differentiable-run entry points check their handle first."""
+    lib = ca.lib()
+    assert lib.cpic_adj_set_field_history(None, 0) == -1
+    assert lib.cpic_adj_backward(None, 1.0, None, None, None, None) == -1
+    assert lib.cpic_adj_state(None, None, None, None, None) == -1
+    assert lib.cpic_adj_device_ms(None, None, None) == -1
